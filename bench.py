@@ -1,0 +1,173 @@
+#!/usr/bin/env python3
+"""Solver benchmark: cell-updates/s of the fused RK2 step on one MI355X.
+
+Workload (the roofline domain): 8192 x 8192 cells, dx = 5 m, seeded fractal
+shelf bed, sediment on (Ashida-Michiue bedload and erosion/deposition),
+N-wave inflow at x = 0, second-order MUSCL (van Leer) + HLL + two-stage RK.
+A cell-update is one cell advanced one full time step (both stages).
+
+Prints one JSON line.  value = FP64 cell-updates/s with the state resident in
+HBM; the FP32-plane variant is reported beside it.  roofline.achieved uses
+algorithmic bytes: 200 B per cell-update in FP64 (stage 1 reads and writes the
+5 planes, stage 2 reads both states and writes U^{n+1}), 100 B in FP32, i.e.
+100 B (FP64) per cell per stage-kernel launch, divided by the stage kernel's
+mean launch duration from HIP events recorded on the solver's stream during
+the timed steps.  cpu_baseline times the CPU oracle (serial FP64
+restatement of the reference arithmetic, 1 thread) on a window of the same
+initial state.
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import platform
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+HBM_PEAK_GBS = 8000.0
+METRIC = "cell-updates/s at 8192² FP64 vs FP32; ensemble sims/hr coarse+NN vs fine"
+
+
+def log(msg):
+    print(msg, file=sys.stderr, flush=True)
+
+
+def cpu_model():
+    try:
+        for line in open("/proc/cpuinfo"):
+            if line.startswith("model name"):
+                return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return platform.processor() or "unknown"
+
+
+def run_variant(sc, precision, steps, warmup, tile_rows):
+    from tsuner_amd.solver import Solver
+    bytes_per_update = 200.0 if precision == "fp64" else 100.0
+    log("[bench] %s: upload %dx%d" % (precision, sc.npx, sc.npy))
+    g = Solver.from_scenario(sc, precision=precision, tile_rows=tile_rows)
+    log("[bench] %s: warmup %d steps (tile rows %d)" % (precision, warmup, g.tile_rows))
+    g.step(warmup)
+    g.set_timing(True)
+    g.timing(reset=True)
+    g.synchronize()
+    t0 = time.perf_counter()
+    g.step(steps, sync=False)
+    g.synchronize()
+    t1 = time.perf_counter()
+    ms, launches = g.timing(reset=True)
+    g.set_timing(False)
+    t, dt, n = g.time()
+    cells = sc.npx * sc.npy
+    wall = t1 - t0
+    nlaunch = launches[0] + launches[1]
+    mean_ms = (ms[0] + ms[1]) / max(nlaunch, 1)
+    per_launch_bytes = bytes_per_update / 2.0 * cells
+    achieved = per_launch_bytes / (mean_ms * 1e-3) / 1e9
+    res = {
+        "value": cells * steps / wall,
+        "ms_per_step": wall * 1e3 / steps,
+        "sim_time": t, "dt": dt, "steps_done": n,
+        "stage_ms_mean": [ms[0] / max(launches[0], 1), ms[1] / max(launches[1], 1)],
+        "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": achieved / HBM_PEAK_GBS, "traffic": None,
+                     "kernel": "stage_kernel<%s, true>" % ("double" if precision == "fp64" else "float"),
+                     "algorithmic_bytes_per_launch": per_launch_bytes},
+        "mass": g.mass(),
+    }
+    log("[bench] %s: %.3f G cell-updates/s, %.3f ms/step, stage kernels %.3f/%.3f ms, %.0f GB/s"
+        % (precision, res["value"] / 1e9, res["ms_per_step"], res["stage_ms_mean"][0], res["stage_ms_mean"][1],
+           achieved))
+    g.close()
+    return res
+
+
+def cpu_baseline(sc, budget_s=12.0):
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import pyoracle as O
+    rows, cols = min(256, sc.npy), min(1024, sc.npx)
+    p = dict(sc.params, npx=cols, npy=rows, nvol=cols * rows, xmax=cols * sc.params["dx"],
+             ymax=rows * sc.params["dy"])
+    s = dict(sc.scheme, bc=[1, 1, 1, 1])
+    V = np.ascontiguousarray(sc.V[:, :rows, :cols])
+    fix = np.ascontiguousarray(sc.fixH[:rows, :cols])
+    o = O.OracleSim(p, s, V, fix)
+    t0 = time.perf_counter()
+    o.step(1)
+    one = time.perf_counter() - t0
+    n = int(max(1, min(200, budget_s / max(one, 1e-6))))
+    t0 = time.perf_counter()
+    o.step(n)
+    el = time.perf_counter() - t0
+    o.close()
+    return {"value": rows * cols * n / el, "unit": "cell-updates/s", "cores": 1, "kind": "port",
+            "sample": "CPU oracle (serial FP64 C restatement, gcc -O2 -ffp-contract=off) on the first %d rows x %d "
+                      "columns of the same initial state (walls), %d steps after 1 warm-up step; host CPU: %s"
+                      % (rows, cols, n, cpu_model())}
+
+
+def main():
+    ap = argparse.ArgumentParser(description=__doc__.split("\n")[0])
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=1000)
+    ap.add_argument("--warmup", type=int, default=20)
+    ap.add_argument("--size", type=int, default=8192)
+    ap.add_argument("--tile-rows", type=int, default=0)
+    ap.add_argument("--no-fp32", action="store_true")
+    ap.add_argument("--no-cpu", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
+        print("bench.py measures one GPU only: this solver path runs on a single MI355X and no multi-GPU "
+              "figure is produced (run with --gpus 1)", file=sys.stderr)
+        sys.exit(2)
+
+    from tsuner_amd import scenarios as S
+    log("[bench] generating roofline domain %d^2" % args.size)
+    t0 = time.perf_counter()
+    sc = S.roofline_domain(n=args.size, dx=5.0 * 8192 / args.size)
+    log("[bench] generated in %.1f s" % (time.perf_counter() - t0))
+
+    r64 = run_variant(sc, "fp64", args.steps, args.warmup, args.tile_rows)
+    r32 = None if args.no_fp32 else run_variant(sc, "fp32", args.steps, args.warmup, args.tile_rows)
+    cpu = None
+    if not args.no_cpu:
+        log("[bench] cpu baseline")
+        cpu = cpu_baseline(sc)
+        log("[bench] cpu %.3e cell-updates/s" % cpu["value"])
+
+    out = {
+        "metric": METRIC,
+        "value": r64["value"],
+        "unit": "cell-updates/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": r64["ms_per_step"],
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64",
+        "data": "synthetic: seeded k^-3 fractal shelf bed, 5% non-erodible mask, 3 m / 600 s N-wave inflow at x=0",
+        "config": {"workload": "roofline domain %dx%d, dx=%g m, sediment on (Ashida-Michiue bedload + "
+                               "erosion/deposition), MUSCL(van Leer) + HLL + RK2, FP64, %d steps"
+                               % (args.size, args.size, sc.params["dx"], args.steps),
+                   "cells": sc.npx * sc.npy, "bytes_per_cell_update": 200},
+        "roofline": r64["roofline"],
+        "cpu_baseline": cpu,
+        "fp32": None if r32 is None else {"value": r32["value"], "ms_per_step": r32["ms_per_step"],
+                                          "dtype": "f32 planes (f64 arithmetic)", "bytes_per_cell_update": 100,
+                                          "roofline": r32["roofline"]},
+        "stage_ms_mean": r64["stage_ms_mean"],
+    }
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

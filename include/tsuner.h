@@ -1,0 +1,184 @@
+/*
+ * tsuner.h -- C ABI of libtsuner, the MI355X (gfx950) implementation of the
+ * TSUNER per-step shallow-water + sediment solver.
+ *
+ * Plain C types only: pointers, sizes, POD structs.  Every entry returns an
+ * int status (TSU_OK = 0); tsu_last_error() gives a message for the calling
+ * thread.  State planes are planar SoA in double precision on the host:
+ * V[f*nvol + j*npx + i] with f = 0 h, 1 qx, 2 qy, 3 H (depth below datum,
+ * bed z = -H), 4 hc (reference layout, GPU/main_TSUNER.cu:104-111).
+ *
+ * Reference interfaces replaced (paths relative to the TSUNER source tree):
+ *   TSUNER()            TSUNER.h:39, GPU/main_TSUNER.cu:21-222
+ *   checkCUDAsupport()  TSUNER.h:41, GPU/CheckCUDASupport.cu:49-73
+ *   initializeGPU()     TSUNER.h:42, GPU/InitializeGPU.cu:21-27
+ *   tsu_state_* / tsu_step / tsu_init_dt split the TSUNER() time loop
+ *     (GPU/main_TSUNER.cu:142-197) into re-entrant calls on an opaque handle.
+ */
+#ifndef TSUNER_AMD_TSUNER_H
+#define TSUNER_AMD_TSUNER_H
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#if defined(TSU_BUILDING_LIBRARY)
+#define TSU_API __attribute__((visibility("default")))
+#else
+#define TSU_API
+#endif
+
+/* CUDA's double2 as the reference's host code uses it (TSUNER.h:39). */
+typedef struct { double x, y; } tsu_double2;
+
+/* Same field order and types as the reference SimParams (TSUNER.h:10-30),
+ * so a caller's existing struct can be passed unchanged.  FR, Q, Opar and
+ * HQB hold the derived values of Problem_loader.cxx:337-342. */
+typedef struct {
+    double xmin, xmax, ymin, ymax;
+    double total_time, save_time;
+    int npx, npy, nvol;
+    double CFL;
+    double Q, R;
+    double porosity;
+    double DS;
+    double FR;
+    double Opar;
+    double OC;
+    double Dsg;
+    double HQB;
+    double vmax;
+    double epsilon_h;
+    double dx;
+    double dy;
+} tsu_sim_params;
+
+/* The reference's compile-time switches (Sim_parameters.hxx:40-54) as
+ * runtime fields, plus the selectable laws.  bc[] order: left (x = xmin),
+ * right (x = xmax), bottom (y = ymin), top (y = ymax). */
+enum { TSU_BC_FREE = 0, TSU_BC_WALL = 1, TSU_BC_PERIODIC = 2, TSU_BC_INFLOW = 3 };
+enum { TSU_LIM_VANLEER = 0, TSU_LIM_MINMOD = 1 };
+enum { TSU_BEDLOAD_ASHIDA_MICHIUE = 0, TSU_BEDLOAD_GRASS = 1 };
+enum { TSU_FRICTION_QUADRATIC = 0, TSU_FRICTION_MANNING = 1 };
+
+typedef struct {
+    int order;          /* 1 forward Euler, 2 two-stage RK (Sim_parameters.hxx:48) */
+    int bedload;        /* BEDLOAD 0/1 (Sim_parameters.hxx:45) */
+    int suspension;     /* SUSPENSION 0/1 (Sim_parameters.hxx:46) */
+    int bc[4];          /* TSU_BC_* per side (WBC, Read_files: Sim_parameters.hxx:51,54) */
+    int limiter;        /* TSU_LIM_* ; reference: van Leer (Reconstructor.cu:19-31) */
+    int bedload_law;    /* TSU_BEDLOAD_* ; reference: Ashida-Michiue (SedFlux.cuh:150-168) */
+    int friction_law;   /* TSU_FRICTION_* ; reference: quadratic (Reconstructor.cu:45-53) */
+    int dt_narrow;      /* 1: dt passes through float for nvol > 4096 (Reduction_kernel_min.cu:46-94) */
+    int extrema;        /* 1: keep running hmax, max H, eta_max (Volume_kernel.cu:79-110) */
+    double grass_ag;    /* Grass coefficient A_g (bedload_law == GRASS) */
+    double manning_n;   /* Manning n (friction_law == MANNING) */
+} tsu_scheme;
+
+enum { TSU_FP64 = 0, TSU_FP32 = 1 };
+
+enum {
+    TSU_OK = 0,
+    TSU_ERR_DEVICE_MEMORY = 1, /* reference code 1: device allocation failed */
+    TSU_ERR_HOST = 2,          /* reference code 2: host allocation / output init failed */
+    TSU_ERR_ARG = 3,
+    TSU_ERR_HIP = 4,
+    TSU_ERR_UNSUPPORTED = 5
+};
+
+/* ---- reference-compatible entries ------------------------------------- */
+
+/* Same arguments, meaning and return codes as TSUNER.h:39 (the reference's
+ * double& tiempo is a pointer at ABI level).  Runs the reference
+ * configuration (second order, bedload and suspension on; inflow series on
+ * all sides when CC/tCC are given, walls otherwise) in FP64 and writes
+ * <input_prefix>.nc.  d, dyrup, time_dyrup and the tides arguments are
+ * accepted and unused, as in the reference build (SPL = 0). */
+TSU_API int TSUNER(double *dVh, tsu_double2 *dVhu, double *dVH, int *fixH, tsu_sim_params params,
+                   char *input_prefix, double *d, double *CC, double *tCC, unsigned long timeCC,
+                   double *dyrup, float *time_dyrup, double *time_gpu, int device,
+                   double *tides_bd, float *tides_time, int tidesNPT);
+
+/* GPU/CheckCUDASupport.cu:49-73: 0 ok, 1 no GPU, 2 no supported GPU
+ * (here: not gfx950), 3 bad device index. */
+TSU_API int checkCUDAsupport(int device);
+TSU_API int tsu_device_check(int device);
+/* GPU/InitializeGPU.cu:21-27: warm-up launch. */
+TSU_API void initializeGPU(void);
+
+/* TSUNER() with an explicit scheme and precision (TSU_FP64 / TSU_FP32). */
+TSU_API int tsu_run(const double *V, const int *fixH, const tsu_sim_params *params, const tsu_scheme *scheme,
+                    int precision, const char *output_prefix, const double *CC, const double *tCC,
+                    unsigned long timeCC, double *time_gpu, int device);
+
+/* Host-only output writer (GPU/netcdf.cu:38-435 variables and attributes):
+ * writes <output_prefix>.nc with nrec snapshots (times[nrec]; V_records holds
+ * nrec consecutive 5*nvol state blocks) and, where non-null, hmax, bmin
+ * (= -bmax, GPU/main_TSUNER.cu:207-209) and eta_max. */
+TSU_API int tsu_write_output(const char *output_prefix, const tsu_sim_params *params, int nrec,
+                             const double *times, const double *V_records, const double *hmax,
+                             const double *bmax, const double *eta_max);
+
+/* ---- handle API --------------------------------------------------------- */
+
+typedef struct tsu_state tsu_state;
+
+TSU_API int tsu_state_create(tsu_state **out, const tsu_sim_params *params, const tsu_scheme *scheme,
+                             int precision, int device);
+TSU_API int tsu_state_destroy(tsu_state *st);
+/* V: 5*nvol doubles (host); fixH: nvol ints in {0, 1} (host). */
+TSU_API int tsu_state_upload(tsu_state *st, const double *V, const int *fixH);
+TSU_API int tsu_state_download(tsu_state *st, double *V);
+/* CC: (2*npx+2*npy)*4*timeCC doubles, tCC: timeCC doubles, layout of
+ * VolumeSolver.cu:172-202; needed when any side is TSU_BC_INFLOW. */
+TSU_API int tsu_state_set_inflow(tsu_state *st, const double *CC, const double *tCC, unsigned long timeCC);
+/* dt of the current state (GPU/main_TSUNER.cu:147-148), kept on device. */
+TSU_API int tsu_init_dt(tsu_state *st);
+/* Enqueue nsteps full time steps; no host synchronisation. */
+TSU_API int tsu_step(tsu_state *st, int nsteps);
+TSU_API int tsu_synchronize(tsu_state *st);
+TSU_API int tsu_get_time(tsu_state *st, double *t, double *dt, long long *steps);
+TSU_API int tsu_set_time(tsu_state *st, double t, double dt);
+/* Steps enqueued after the simulation time reaches stop_time do nothing. */
+TSU_API int tsu_set_stop_time(tsu_state *st, double stop_time);
+/* FP64 sums of h and of (H + hc) over all cells. */
+TSU_API int tsu_mass(tsu_state *st, double out[2]);
+TSU_API int tsu_get_extrema(tsu_state *st, double *hmax, double *bmax, double *eta_max);
+/* Rows marched by one wave (tile height); 0 picks the default. */
+TSU_API int tsu_set_tile_rows(tsu_state *st, int rows);
+TSU_API int tsu_get_tile_rows(tsu_state *st);
+/* Replay steps through a captured HIP graph of chunk steps (0 disables). */
+TSU_API int tsu_set_graph_chunk(tsu_state *st, int chunk);
+/* hipEvent timing of every stage-kernel launch enqueued while on (eager
+ * steps only; graph replays are not timed).  tsu_get_timing synchronises and
+ * returns the summed milliseconds and launch counts per RK stage (arrays of
+ * 2); reset != 0 clears the accumulators. */
+TSU_API int tsu_set_timing(tsu_state *st, int on);
+TSU_API int tsu_get_timing(tsu_state *st, double stage_ms[2], long long launches[2], int reset);
+/* The stream all work of this handle is enqueued on (a hipStream_t). */
+TSU_API void *tsu_stream(tsu_state *st);
+/* Device pointer of the U^n planes (element size 8 or 4 by precision). */
+TSU_API void *tsu_device_state(tsu_state *st);
+
+TSU_API const char *tsu_last_error(void);
+TSU_API const char *tsu_version(void);
+
+/* ---- known-answer entries (device functions on arrays of inputs) -------- */
+
+/* n edges; left/right: 5 doubles each (h, qx, qy, H, hc, unrotated);
+ * normal: 2 ints each, one of (1,0) (-1,0) (0,1) (0,-1); fixed: n ints.
+ * Dm, Dp: 5 doubles each, unrotated (VolumeSolver.cu:19-170). */
+TSU_API int tsu_kat_edge_flux(const tsu_sim_params *params, const tsu_scheme *scheme, int n,
+                              const double *left, const double *right, const int *normal,
+                              const int *fixed, double *Dm, double *Dp, int device);
+/* what: 0 limiter(a,b) -> o1; 1 velocity(h=a,qx=b,qy=c) -> o1,o2,o3=|u|;
+ * 2 state filter(a,b,c,d) -> o1..o4; 3 cell dt(a,b,c) -> o1;
+ * 4 erosion rate(h=a,qx=b,qy=c,hc=d) -> o1; 5 bedload(h=a,un=b,ut=c) -> o1. */
+TSU_API int tsu_kat_point(const tsu_sim_params *params, const tsu_scheme *scheme, int what, int n,
+                          const double *a, const double *b, const double *c, const double *d,
+                          double *o1, double *o2, double *o3, double *o4, int device);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

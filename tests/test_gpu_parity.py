@@ -1,0 +1,284 @@
+"""Parity of the HIP path (through the libtsuner C ABI) with the CPU oracle.
+
+FP64 results must equal the oracle's as values (+0 == -0): the kernels keep
+the reference's operation order and are built without FMA contraction.
+"""
+import sys
+
+import numpy as np
+import pytest
+
+from tsuner_amd import scenarios as S
+from tsuner_amd import _lib
+from tsuner_amd.solver import Solver
+
+pytestmark = pytest.mark.gpu
+
+
+def say(msg):
+    print(msg, flush=True)
+    sys.stdout.flush()
+
+
+def cmp_state(a, b, what=""):
+    if not np.array_equal(a, b):
+        d = np.abs(a - b)
+        f, j, i = np.unravel_index(np.nanargmax(d), d.shape)
+        raise AssertionError("%s: max |diff| %.3e at plane %d cell (%d,%d): %r vs %r; %d cells differ"
+                             % (what, d[f, j, i], f, j, i, a[f, j, i], b[f, j, i], int((a != b).sum())))
+
+
+def run_pair(sc, steps, chunks=1, tile_rows=0, tol=None):
+    import pyoracle as O
+    say("case %s steps=%d tile_rows=%d" % (sc.name, steps, tile_rows))
+    o = O.from_scenario(sc)
+    g = Solver.from_scenario(sc, tile_rows=tile_rows)
+    assert g.dt == o.dt, "initial dt %r vs %r" % (g.dt, o.dt)
+    per = steps // chunks
+    for c in range(chunks):
+        o.step(per)
+        g.step(per)
+        t, dt, n = g.time()
+        if tol is None:
+            cmp_state(o.state(), g.state(), "%s after %d steps" % (sc.name, (c + 1) * per))
+            assert t == o.t and dt == o.dt, (t, o.t, dt, o.dt)
+        else:
+            a, b = o.state(), g.state()
+            scale = np.maximum(np.abs(a).max(axis=(1, 2), keepdims=True), 1e-30)
+            err = (np.abs(a - b) / scale).max()
+            assert err <= tol, "%s relative error %.3e > %.1e" % (sc.name, err, tol)
+        assert n == (c + 1) * per
+    return o, g
+
+
+# ---------------------------------------------------------------- KATs ----
+
+def random_faces(rng, n):
+    h = rng.uniform(0, 10, n)
+    r = rng.uniform(0, 1, n)
+    h[r < 0.25] = 0.0
+    sel = (r >= 0.25) & (r < 0.35)
+    h[sel] = rng.uniform(0, 2e-6, sel.sum())
+    h[(r >= 0.35) & (r < 0.37)] = 1e-6
+    H = rng.uniform(-5, 20, n)
+    qx = rng.normal(0, 15, n)
+    qy = rng.normal(0, 15, n)
+    c = rng.uniform(-0.01, 0.05, n)
+    hc = h * c
+    return np.stack([h, qx, qy, H, hc], axis=1)
+
+
+@pytest.mark.parametrize("law", [S.BEDLOAD_AM, S.BEDLOAD_GRASS])
+def test_kat_edge_flux(oracle, law):
+    rng = np.random.default_rng(11 + law)
+    n = 3000
+    L, R = random_faces(rng, n), random_faces(rng, n)
+    # shallow subset so bedload thresholds (h > 5 DS) and theta > theta_c are crossed
+    L[:500, 0] = rng.uniform(0, 3e-3, 500)
+    R[:500, 0] = rng.uniform(0, 3e-3, 500)
+    normals = np.array([(1, 0), (-1, 0), (0, 1), (0, -1)], dtype=np.int32)[rng.integers(0, 4, n)]
+    fixed = rng.integers(0, 2, n).astype(np.int32)
+    sc = S.sediment_shelf(n=8, dx=10.0, inflow=False, bedload_law=law)
+    p, s = sc.params, sc.scheme
+    Dm = np.zeros((n, 5))
+    Dp = np.zeros((n, 5))
+    L_ = np.ascontiguousarray(L)
+    R_ = np.ascontiguousarray(R)
+    lib = _lib.lib()
+    _lib.check(lib.tsu_kat_edge_flux(_lib.make_params(p), _lib.make_scheme(s), n, _lib.ptr(L_), _lib.ptr(R_),
+                                     _lib.ptr(np.ascontiguousarray(normals), _lib.C.c_int),
+                                     _lib.ptr(fixed, _lib.C.c_int), _lib.ptr(Dm), _lib.ptr(Dp), 0), "kat")
+    for i in range(n):
+        dm, dp = oracle.edge_flux(p, s, L[i], R[i], normals[i], fixed[i])
+        if not (np.array_equal(dm, Dm[i]) and np.array_equal(dp, Dp[i])):
+            raise AssertionError("edge %d: L=%r R=%r n=%r fixed=%d\n oracle Dm=%r Dp=%r\n gpu Dm=%r Dp=%r"
+                                 % (i, L[i], R[i], normals[i], fixed[i], dm, dp, Dm[i], Dp[i]))
+
+
+def kat_point(p, s, what, a, b=None, c=None, d=None):
+    n = len(a)
+    arr = [np.ascontiguousarray(x, dtype=np.float64) if x is not None else None for x in (a, b, c, d)]
+    outs = [np.zeros(n) for _ in range(4)]
+    lib = _lib.lib()
+    _lib.check(lib.tsu_kat_point(_lib.make_params(p), _lib.make_scheme(s), what, n,
+                                 *[(_lib.ptr(x) if x is not None else None) for x in arr],
+                                 *[_lib.ptr(o) for o in outs], 0), "kat_point")
+    return outs
+
+
+def test_kat_points(oracle):
+    rng = np.random.default_rng(5)
+    sc = S.sediment_shelf(n=8, dx=10.0, inflow=False)
+    p, s = sc.params, sc.scheme
+    n = 4000
+    F = random_faces(rng, n)
+    h, qx, qy, H, hc = F.T
+    for law in (S.LIM_VANLEER, S.LIM_MINMOD):
+        a = rng.normal(0, 1, n) * 10.0 ** rng.integers(-18, 2, n)
+        b = rng.normal(0, 1, n) * 10.0 ** rng.integers(-18, 2, n)
+        a[:50] = 0.0
+        b[25:75] = -a[25:75]
+        s2 = dict(s, limiter=law)
+        o1 = kat_point(p, s2, 0, a, b)[0]
+        ref = np.array([oracle.limiter(law, x, y) for x, y in zip(a, b)])
+        np.testing.assert_array_equal(o1, ref)
+    ux, uy, m, _ = kat_point(p, s, 1, h, qx, qy)
+    ref = np.array([oracle.velocity(*v) for v in zip(h, qx, qy)])
+    np.testing.assert_array_equal(ux, ref[:, 0])
+    np.testing.assert_array_equal(uy, ref[:, 1])
+    np.testing.assert_array_equal(m, np.sqrt(ref[:, 0] ** 2 + ref[:, 1] ** 2))
+    o = kat_point(p, s, 2, h - 0.5, qx, qy, hc)
+    ref = np.array([oracle.state_filter(*v) for v in zip(h - 0.5, qx, qy, hc)])
+    for k in range(4):
+        np.testing.assert_array_equal(o[k], ref[:, k])
+    o1 = kat_point(p, s, 3, h, qx, qy)[0]
+    ref = np.array([oracle.cell_dt(p, *v) for v in zip(h, qx, qy)])
+    np.testing.assert_array_equal(o1, ref)
+    o1 = kat_point(p, s, 4, h, qx, qy, np.abs(hc))[0]
+    ref = np.array([oracle.erosion_rate(p, s, *v) for v in zip(h, qx, qy, np.abs(hc))])
+    np.testing.assert_array_equal(o1, ref)
+    un = rng.normal(0, 2, n)
+    ut = rng.normal(0, 2, n)
+    hh = rng.uniform(0, 3e-3, n)
+    for law in (S.BEDLOAD_AM, S.BEDLOAD_GRASS):
+        s2 = dict(s, bedload_law=law)
+        o1 = kat_point(p, s2, 5, hh, un, ut)[0]
+        ref = np.array([oracle.bedload_flux(p, s2, *v) for v in zip(hh, un, ut)])
+        np.testing.assert_array_equal(o1, ref)
+
+
+# ------------------------------------------------------- grid parity -------
+
+def test_plain_swe_exact_dt_path():
+    run_pair(S.plain_swe(n=64, dx=160.0), 40, chunks=2)
+
+
+def test_plain_swe_narrowed_dt_multi_strip():
+    # 130 x 70 cells: three 60-column strips, 9100 cells (float-narrowed dt)
+    sc = S.plain_swe(n=130, ny=70, dx=80.0)
+    run_pair(sc, 30, chunks=3)
+
+
+def test_sediment_inflow_odd_size():
+    sc = S.sediment_shelf(n=67, ny=37, dx=40.0)
+    run_pair(sc, 40, chunks=2, tile_rows=5)
+
+
+def test_sediment_inflow_all_sides():
+    sc = S.sediment_shelf(n=61, ny=33, dx=40.0)
+    sc.scheme["bc"] = [S.BC_INFLOW] * 4
+    run_pair(sc, 25)
+
+
+def test_periodic_both_axes():
+    sc = S.sediment_shelf(n=70, ny=45, dx=30.0, inflow=False, seed=9)
+    sc.scheme["bc"] = [S.BC_PERIODIC] * 4
+    sc.V[1] += 1.0   # some flow across the periodic seams
+    run_pair(sc, 30, tile_rows=7)
+
+
+def test_free_boundaries_and_extrema():
+    sc = S.gaussian_source(n=64, dx=30.0, seed=3)
+    sc.scheme["bc"] = [S.BC_FREE, S.BC_WALL, S.BC_FREE, S.BC_WALL]
+    sc.scheme["extrema"] = 1
+    o, g = run_pair(sc, 30)
+    for a, b in zip(o.extrema(), g.extrema()):
+        np.testing.assert_array_equal(a, b)
+
+
+def test_first_order():
+    sc = S.sediment_shelf(n=50, ny=41, dx=40.0)
+    sc.scheme["order"] = 1
+    run_pair(sc, 31, chunks=1)
+
+
+def test_minmod_grass():
+    sc = S.sediment_shelf(n=65, ny=30, dx=40.0, limiter=S.LIM_MINMOD, bedload_law=S.BEDLOAD_GRASS)
+    run_pair(sc, 30)
+
+
+def test_manning_friction():
+    # cbrt is not correctly rounded on either side: tolerance, not equality
+    sc = S.plain_swe(n=64, dx=160.0, friction_law=S.FRIC_MANNING, manning_n=0.025)
+    run_pair(sc, 30, tol=1e-11)
+
+
+@pytest.mark.parametrize("rows", [1, 3, 16, 64])
+def test_tile_height_independent(rows):
+    sc = S.sediment_shelf(n=75, ny=50, dx=40.0, seed=4)
+    run_pair(sc, 12, tile_rows=rows)
+
+
+def test_graph_replay_matches_eager():
+    sc = S.sediment_shelf(n=90, ny=60, dx=40.0, seed=6)
+    a = Solver.from_scenario(sc)
+    b = Solver.from_scenario(sc)
+    b.set_graph_chunk(10)
+    for k in range(3):
+        a.step(10)
+        b.step(10)
+        cmp_state(a.state(), b.state(), "graph replay %d" % k)
+        assert a.time() == b.time()
+        b.mass()   # eager work between replays
+
+
+def test_stop_time_makes_steps_noops():
+    sc = S.plain_swe(n=64, dx=160.0)
+    g = Solver.from_scenario(sc)
+    g.step(5)
+    t5 = g.t
+    g.set_stop_time(t5)
+    g.step(7)
+    assert g.steps == 5 and g.t == t5
+    g.set_stop_time(float("inf"))
+    g.step(1)
+    assert g.steps == 6
+
+
+# ---------------------------------------------- full-size invariants --------
+
+def test_lake_at_rest_2048():
+    sc = S.lake_at_rest(n=2048)
+    say("lake at rest 2048^2")
+    g = Solver.from_scenario(sc)
+    g.step(5)
+    cmp_state(sc.V, g.state(), "lake at rest")
+
+
+def test_mass_conservation_1024():
+    sc = S.sediment_shelf(n=1024, dx=10.0, inflow=False, seed=8)
+    sc.V[0] = np.maximum(0.0, sc.V[3] + 1.5 * np.exp(-((np.arange(1024) - 300.0) ** 2) / 2e3)[None, :])
+    g = Solver.from_scenario(sc)
+    m0 = g.mass()
+    g.step(50)
+    m1 = g.mass()
+    assert abs(m1[0] - m0[0]) <= 1e-11 * m0[0], (m0, m1)
+    assert abs(m1[1] - m0[1]) <= 1e-11 * abs(m0[1]), (m0, m1)
+
+
+def test_transpose_symmetry_512():
+    sc = S.gaussian_source(n=256, dx=40.0, seed=12)
+    sct = S.gaussian_source(n=256, dx=40.0, seed=12)
+    sct.V = np.ascontiguousarray(np.stack([sc.V[0].T, sc.V[2].T, sc.V[1].T, sc.V[3].T, sc.V[4].T]))
+    sct.fixH = np.ascontiguousarray(sc.fixH.T)
+    a = Solver.from_scenario(sc)
+    b = Solver.from_scenario(sct)
+    a.step(40)
+    b.step(40)
+    A, B = a.state(), b.state()
+    Bt = np.stack([B[0].T, B[2].T, B[1].T, B[3].T, B[4].T])
+    cmp_state(A, Bt, "transpose symmetry")
+
+
+def test_fp32_planes_close_to_fp64(oracle):
+    sc = S.sediment_shelf(n=96, ny=64, dx=40.0, seed=10)
+    o = oracle.from_scenario(sc)
+    g = Solver.from_scenario(sc, precision="fp32")
+    o.step(20)
+    g.step(20)
+    a, b = o.state(), g.state()
+    for f, name in enumerate(["h", "qx", "qy", "H", "hc"]):
+        scale = max(np.abs(a[f]).max(), 1e-12)
+        err = np.abs(a[f] - b[f]).max() / scale
+        say("fp32 rel err %s %.3e" % (name, err))
+        assert err < 1e-4, (name, err)

@@ -1,0 +1,50 @@
+// tsu_kernels.h -- host-side launchers of the gfx950 kernels (internal).
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "tsu_device.h"
+
+namespace tsu {
+
+// Columns of output per wave: a 64-lane wave covers 64 consecutive columns,
+// the two outer lanes on each side are halo.
+constexpr int kStripOut = 60;
+
+struct StageArgs {
+    const void* in;       // stage input planes (h, qx, qy, H, hc), stride nvol
+    const void* Hin;      // bed plane read by the stencil when sediment is off
+    const void* v0;       // U^n planes, read at own cells only (may alias out)
+    void* out;            // output planes
+    int pingpong;         // 1: on odd steps in and out swap roles (single-stage scheme)
+    const uint8_t* fix;   // 1 = erodible bed, 0 = fixed
+    Ctrl* ctrl;
+    double* ext;          // hmax, max H, eta_max planes (nvol each) or null
+    int k;                // RK stage index
+    int last;             // 1 on the last stage of a step (dt epilogue)
+    int tile_rows;
+    int nstrips;
+    int ntiles;
+};
+
+// precision: 0 = FP64 planes, 1 = FP32 planes (FP64 arithmetic).
+hipError_t launch_stage(const StageArgs& a, const DevParams& P, bool sed, int precision, hipStream_t s);
+hipError_t launch_init_dt(const void* V, const DevParams& P, Ctrl* ctrl, int precision, hipStream_t s);
+hipError_t launch_warmup(hipStream_t s);
+hipError_t launch_advance(Ctrl* ctrl, const DevParams& P, hipStream_t s);
+hipError_t launch_fill(double* p, long long n, double v, hipStream_t s);
+// Sums of h and of (H + hc) in FP64 into out[0], out[1] (device pointer),
+// using partial (>= 2 * nblocks doubles) as scratch.
+hipError_t launch_mass(const void* V, long long nvol, int precision, double* partial, int nblocks, double* out,
+                       hipStream_t s);
+int mass_blocks(long long nvol);
+
+// Known-answer kernels running the device functions on arrays of inputs.
+hipError_t launch_kat_edge(const DevParams& P, int n, const double* left, const double* right, const int* kind,
+                           const int* fixed, double* Dm, double* Dp, hipStream_t s);
+hipError_t launch_kat_point(int what, int law, int n, const double* a, const double* b, const double* c,
+                            const double* d, double* o1, double* o2, double* o3, double* o4, const DevParams& P,
+                            hipStream_t s);
+
+}  // namespace tsu

@@ -1,0 +1,648 @@
+// tsu_kernels.hip -- gfx950 kernels of the TSUNER per-step solver.
+//
+// One launch of stage_kernel performs a whole RK stage of the reference
+// pipeline (reconstructor -> VolumeSolver -> states_updater,
+// GPU/main_TSUNER.cu:170-183) for a tile of cells, with the reference's
+// face arrays reconsL/R/D/U and edge slots FL/FR/FD/FU kept in registers:
+//
+//   * a 64-lane wave owns 64 consecutive columns (lanes 2..61 are outputs,
+//     lanes 0,1,62,63 are halo) and marches over tile_rows rows;
+//   * x neighbours are exchanged with lane shuffles, y neighbours stay in a
+//     three-row register window, so each cell's velocity, each face and each
+//     edge flux is evaluated exactly once per stage;
+//   * the last stage of a step also computes the per-cell CFL dt
+//     (Volume_kernel.cu:19-43), reduces it over the wave with shuffles and
+//     folds it into the device-resident step control block with one 64-bit
+//     atomicMin per wave (Reduction_kernel_min.cu:155-196), so dt never
+//     leaves the device.
+//
+// Workgroups are dealt round-robin over the 8 XCDs; tile ids are remapped so
+// every XCD receives a contiguous band of tiles (neighbouring tiles share
+// halo rows and partial cache lines through that XCD's L2).
+#include <hip/hip_runtime.h>
+
+#include "tsu_device.h"
+#include "tsu_kernels.h"
+
+namespace tsu {
+
+namespace {
+
+constexpr unsigned long long kInfBits = 0x7FF0000000000000ull;
+
+__device__ __forceinline__ int map_index(int i, int n, bool periodic)
+{
+    if (periodic) {
+        i %= n;
+        return i < 0 ? i + n : i;
+    }
+    return min(max(i, 0), n - 1);
+}
+
+__device__ __forceinline__ int xcd_tile(int b, int nb)
+{
+    const int q = nb >> 3, r = nb & 7, x = b & 7;
+    return x * q + min(x, r) + (b >> 3);
+}
+
+__device__ __forceinline__ double up1(double v) { return __shfl_up(v, 1, 64); }
+__device__ __forceinline__ double dn1(double v) { return __shfl_down(v, 1, 64); }
+__device__ __forceinline__ int up1i(int v) { return __shfl_up(v, 1, 64); }
+
+__device__ __forceinline__ double wave_min(double v)
+{
+    for (int o = 32; o > 0; o >>= 1) v = fmin(v, __shfl_xor(v, o, 64));
+    return v;
+}
+
+struct Cell {
+    double h, qx, qy, H, hc;   // state
+    double ux, uy, c;          // velocity and concentration
+};
+
+template <typename S, bool SED>
+__device__ __forceinline__ void load_cell(const S* in, const S* Hin, long long nv, long long off, Cell& v)
+{
+    v.h = (double)in[off];
+    v.qx = (double)in[nv + off];
+    v.qy = (double)in[2 * nv + off];
+    v.H = (double)Hin[off];
+    v.hc = SED ? (double)in[4 * nv + off] : 0.0;
+}
+
+template <bool SED>
+__device__ __forceinline__ void derive_cell(Cell& v)
+{
+    Vel u = velocity(v.h, v.qx, v.qy);
+    v.ux = u.x;
+    v.uy = u.y;
+    v.c = SED ? v.hc / (v.h + kEpsilon) : 0.0;
+}
+
+// Reconstructor.cu:146-194 along one axis: faces on the minus side (Fm) and
+// plus side (Fp) of the centre cell from its minus (l) and plus (r)
+// neighbours; returns the limited free-surface slope used by the
+// non-conservative source (Reconstructor.cu:197,274).
+template <bool SED>
+__device__ __forceinline__ double recons_axis(int lim, const Cell& ci, double hl, double Hl, double cl, double ulx,
+                                              double uly, double hr, double Hr, double cr, double urx, double ury,
+                                              double* Fm, double* Fp)
+{
+    const double hi = ci.h, Hi = ci.H;
+    double dl = hi - hl, dr = hr - hi;
+    double dh = limiter(lim, dl, dr);
+    if (0.5 * fabs(dh) > hi) dh = 2.0 * hi * (double)sgn(dh);
+    Fm[0] = hi - 0.5 * dh;
+    Fp[0] = hi + 0.5 * dh;
+
+    double zm = fmin(Hl, Hi);
+    dl = fmax(hi - Hi + zm, 0.0) - fmax(hl - Hl + zm, 0.0);
+    zm = fmin(Hr, Hi);
+    dr = fmax(hr - Hr + zm, 0.0) - fmax(hi - Hi + zm, 0.0);
+    double deta = limiter(lim, dl, dr);
+    double dz = dh - deta;
+    Fm[3] = Hi - 0.5 * dz;
+    Fp[3] = Hi + 0.5 * dz;
+
+    double du = limiter(lim, ci.ux - ulx, urx - ci.ux);
+    double dq = hi * du + ci.ux * dh;
+    Fm[1] = ci.qx - 0.5 * dq;
+    Fp[1] = ci.qx + 0.5 * dq;
+
+    du = limiter(lim, ci.uy - uly, ury - ci.uy);
+    dq = hi * du + ci.uy * dh;
+    Fm[2] = ci.qy - 0.5 * dq;
+    Fp[2] = ci.qy + 0.5 * dq;
+
+    if (SED) {
+        du = limiter(lim, ci.c - cl, cr - ci.c);
+        Fm[4] = (ci.c - 0.5 * du) * Fm[0];
+        Fp[4] = (ci.c + 0.5 * du) * Fp[0];
+    } else {
+        Fm[4] = Fp[4] = 0.0;
+    }
+    return deta;
+}
+
+template <bool SED>
+__device__ __forceinline__ void filter_face(double* F)
+{
+    state_filter<SED>(F[0], F[1], F[2], F[4]);
+}
+
+__device__ __forceinline__ void cell_face(const Cell& c, double* F)
+{
+    F[0] = c.h;
+    F[1] = c.qx;
+    F[2] = c.qy;
+    F[3] = c.H;
+    F[4] = c.hc;
+}
+
+// Riemann problem on one edge with left/right faces in the unrotated frame;
+// Dm, Dp come back in the unrotated frame (VolumeSolver.cu:19-170).
+template <bool SED>
+__device__ __forceinline__ void edge_flux(const DevParams& P, int kind, bool mobile, const double* Lf,
+                                          const double* Rf, double* Dm, double* Dp)
+{
+    double qnl, qtl, qnr, qtr, dm[5], dp[5];
+    to_normal(kind, Lf[1], Lf[2], qnl, qtl);
+    to_normal(kind, Rf[1], Rf[2], qnr, qtr);
+    riemann<SED>(P, mobile, Lf[0], Lf[3], qnl, qtl, Lf[4], Rf[0], Rf[3], qnr, qtr, Rf[4], dm, dp);
+    Dm[0] = dm[0];
+    Dp[0] = dp[0];
+    from_normal(kind, dm[1], dm[2], Dm[1], Dm[2]);
+    from_normal(kind, dp[1], dp[2], Dp[1], Dp[2]);
+    if (SED) {
+        Dm[3] = dm[3];
+        Dm[4] = dm[4];
+        Dp[3] = dp[3];
+        Dp[4] = dp[4];
+    } else {
+        Dm[3] = Dm[4] = Dp[3] = Dp[4] = 0.0;
+    }
+}
+
+template <typename S, bool SED>
+__global__ __launch_bounds__(64) void stage_kernel(StageArgs a, DevParams P)
+{
+    const int lane = threadIdx.x;
+    const int tile = xcd_tile(blockIdx.x, gridDim.x);
+    const int strip = tile % a.nstrips, chunk = tile / a.nstrips;
+
+    Ctrl* C = a.ctrl;
+    const double ct = C->t;
+    if (!(ct < C->stop_time)) return;
+    const long long step = C->step;
+    const double dt = finalize_dt(P, C->dt_bits[step & 1]);
+
+    const int npx = P.npx, npy = P.npy;
+    const long long nv = P.nvol;
+    const S* in = (const S*)a.in;
+    S* out = (S*)a.out;
+    if (a.pingpong && (step & 1)) {
+        in = (const S*)a.out;
+        out = (S*)a.in;
+    }
+    const S* Hin = SED ? in + 3 * nv : (const S*)a.Hin;
+    const S* v0p = (a.k == 0) ? in : (const S*)a.v0;
+    const uint8_t* fix = a.fix;
+
+    const bool perx = P.bc[0] == BC_PERIODIC, pery = P.bc[2] == BC_PERIODIC;
+    const int x = strip * kStripOut + lane - 2;
+    const int xc = map_index(x, npx, perx);
+    const int y0 = chunk * a.tile_rows;
+    const int y1 = min(y0 + a.tile_rows, npy);
+    const bool out_lane = lane >= 2 && lane < 62 && x < npx;
+    const int lim = P.limiter;
+    const bool second = P.order == 2;
+    const bool need_fix = SED && P.bedload;
+
+    // Per-step constants, in the reference's evaluation order.
+    const double invx = dt * P.inv_dx, invy = dt * P.inv_dy;  // main_TSUNER.cu:182
+    const double dtaxG = invx * kG, dtayG = invy * kG;        // Reconstructor.cu:197,274
+    const double dtFR = dt * P.FR;                             // Reconstructor.cu:45
+    const double dtGnn = dt * kG * P.manning_n * P.manning_n;
+
+    auto row_off = [&](int r) -> long long { return (long long)map_index(r, npy, pery) * npx + xc; };
+    auto fix_at = [&](int r) -> int { return need_fix ? (int)fix[row_off(r)] : 0; };
+
+    // ---- prologue: faces of row y0 and the edge y0 - 1/2 -----------------
+    Cell Rm, R0, R1;
+    load_cell<S, SED>(in, Hin, nv, row_off(y0 - 1), Rm);
+    load_cell<S, SED>(in, Hin, nv, row_off(y0), R0);
+    load_cell<S, SED>(in, Hin, nv, row_off(y0 + 1), R1);
+    derive_cell<SED>(Rm);
+    derive_cell<SED>(R0);
+    derive_cell<SED>(R1);
+    int fx0 = fix_at(y0), fx1;
+
+    double D0[5], U0[5], FD0[5];
+    double detay0 = 0.0;
+    if (second) {
+        detay0 = recons_axis<SED>(lim, R0, Rm.h, Rm.H, Rm.c, Rm.ux, Rm.uy, R1.h, R1.H, R1.c, R1.ux, R1.uy, D0, U0);
+        filter_face<SED>(D0);
+        filter_face<SED>(U0);
+    } else {
+        cell_face(R0, D0);
+        cell_face(R0, U0);
+    }
+    if (y0 == 0 && !pery) {
+        double G5[5], dp[5];
+        ghost_state(P, 2, D0, G5, (long long)xc, 2, ct);
+        edge_flux<SED>(P, N_MY, fx0 != 0, D0, G5, FD0, dp);
+    } else {
+        double Um[5], Dd[5], dm[5];
+        if (second) {
+            Cell Rmm;
+            load_cell<S, SED>(in, Hin, nv, row_off(y0 - 2), Rmm);
+            derive_cell<SED>(Rmm);
+            recons_axis<SED>(lim, Rm, Rmm.h, Rmm.H, Rmm.c, Rmm.ux, Rmm.uy, R0.h, R0.H, R0.c, R0.ux, R0.uy, Dd, Um);
+            filter_face<SED>(Um);
+        } else {
+            cell_face(Rm, Um);
+        }
+        int fxm = fix_at(y0 - 1);
+        int fixed = (y0 == 0) ? fxm : fxm * fx0;   // periodic wrap edge: VolumeSolver.cu:386
+        edge_flux<SED>(P, N_PY, fixed != 0, Um, D0, dm, FD0);
+    }
+
+    double dtmin = __longlong_as_double((long long)kInfBits);
+
+    // ---- march over the rows of the tile -----------------------------------
+    for (int r = y0; r < y1; ++r) {
+        // next row (r + 2): issued first, consumed after the x work below
+        Cell R2;
+        load_cell<S, SED>(in, Hin, nv, row_off(r + 2), R2);
+        fx1 = fix_at(r + 1);
+        double v0[5];
+        const long long pos = (long long)r * npx + xc;
+        if (a.k != 0 && out_lane) {
+            v0[0] = (double)v0p[pos];
+            v0[1] = (double)v0p[nv + pos];
+            v0[2] = (double)v0p[2 * nv + pos];
+            v0[3] = SED ? (double)v0p[3 * nv + pos] : R0.H;
+            v0[4] = SED ? (double)v0p[4 * nv + pos] : 0.0;
+        }
+
+        // x faces of row r (Reconstructor.cu:120-197)
+        double hl = up1(R0.h), Hl = up1(R0.H), ulx = up1(R0.ux), uly = up1(R0.uy);
+        double hr = dn1(R0.h), Hr = dn1(R0.H), urx = dn1(R0.ux), ury = dn1(R0.uy);
+        double cl = 0.0, cr = 0.0;
+        if (SED) {
+            cl = up1(R0.c);
+            cr = dn1(R0.c);
+        }
+        double Lf[5], Rf[5], detax = 0.0;
+        if (second) {
+            detax = recons_axis<SED>(lim, R0, hl, Hl, cl, ulx, uly, hr, Hr, cr, urx, ury, Lf, Rf);
+            filter_face<SED>(Lf);
+            filter_face<SED>(Rf);
+        } else {
+            cell_face(R0, Lf);
+            cell_face(R0, Rf);
+        }
+
+        // x edge at x - 1/2 (VolumeSolver.cu:227-317)
+        double Rl[5];
+        for (int f = 0; f < 5; f++) Rl[f] = up1(Rf[f]);
+        int fxl = need_fix ? up1i(fx0) : 0;
+        double El[5], Er[5];
+        int kind = N_PX;
+        int fixed;
+        if (x == 0 && !perx) {
+            for (int f = 0; f < 5; f++) El[f] = Lf[f];
+            ghost_state(P, 0, Lf, Er, 2LL * npx + npy + r, 1, ct);
+            kind = N_MX;
+            fixed = fx0;
+        } else if (x == npx && !perx) {
+            for (int f = 0; f < 5; f++) El[f] = Rl[f];
+            ghost_state(P, 1, Rl, Er, (long long)npx + r, 1, ct);
+            fixed = fxl;
+        } else {
+            for (int f = 0; f < 5; f++) {
+                El[f] = Rl[f];
+                Er[f] = Lf[f];
+            }
+            fixed = (perx && (x == 0 || x == npx)) ? fxl : fxl * fx0;
+        }
+        double Dmx[5], Dpx[5];
+        edge_flux<SED>(P, kind, fixed != 0, El, Er, Dmx, Dpx);
+        double FL[5], FR[5];
+        const bool left_wall = (x == 0 && !perx);
+        for (int f = 0; f < 5; f++) {
+            FL[f] = left_wall ? Dmx[f] : Dpx[f];
+            FR[f] = dn1(Dmx[f]);
+        }
+
+        // y faces of row r + 1 and the edge r + 1/2
+        derive_cell<SED>(R2);
+        double D1[5], U1[5], detay1 = 0.0;
+        if (second) {
+            detay1 = recons_axis<SED>(lim, R1, R0.h, R0.H, R0.c, R0.ux, R0.uy, R2.h, R2.H, R2.c, R2.ux, R2.uy, D1, U1);
+            filter_face<SED>(D1);
+            filter_face<SED>(U1);
+        } else {
+            cell_face(R1, D1);
+            cell_face(R1, U1);
+        }
+        double FU0[5], FD1[5];
+        if (r + 1 == npy && !pery) {
+            double G5[5], dp[5];
+            ghost_state(P, 3, U0, G5, (long long)npx + npy + xc, 2, ct);
+            edge_flux<SED>(P, N_PY, fx0 != 0, U0, G5, FU0, dp);
+        } else {
+            int fy = (r + 1 == npy) ? fx0 : fx0 * fx1;
+            edge_flux<SED>(P, N_PY, fy != 0, U0, D1, FU0, FD1);
+        }
+
+        // cell update of row r (Reconstructor.cu:197,274,353-354;
+        // Volume_kernel.cu:191-195,244-282)
+        if (out_lane) {
+            if (a.k == 0) {
+                v0[0] = R0.h;
+                v0[1] = R0.qx;
+                v0[2] = R0.qy;
+                v0[3] = R0.H;
+                v0[4] = R0.hc;
+            }
+            double k1 = R0.qx, k2 = R0.qy, k3 = R0.H, k4 = R0.hc;
+            if (second) {
+                k1 -= dtaxG * R0.h * detax;
+                k2 -= dtayG * R0.h * detay0;
+            }
+            Vel un = velocity(v0[0], v0[1], v0[2]);
+            double cf;
+            if (P.friction_law == FRIC_MANNING) {
+                double hp = v0[0] + kEpsilon;
+                cf = dtGnn * un.m / (hp * cbrt(hp));
+            } else {
+                cf = dtFR * un.m / (v0[0] + kEpsilon);
+            }
+            k1 /= (1.0 + cf);
+            k2 /= (1.0 + cf);
+            if (SED && P.suspension) {
+                double phi = erosion_rate(P, v0[0], un, v0[4]);
+                k3 -= dt * phi;
+                k4 += dt * phi;
+            }
+            double W[5];
+            W[0] = R0.h - (invx * (FL[0] + FR[0]) + invy * (FD0[0] + FU0[0]));
+            W[1] = k1 - (invx * (FL[1] + FR[1]) + invy * (FD0[1] + FU0[1]));
+            W[2] = k2 - (invx * (FL[2] + FR[2]) + invy * (FD0[2] + FU0[2]));
+            if (SED) {
+                W[3] = k3 - (invx * (FL[3] + FR[3]) + invy * (FD0[3] + FU0[3]));
+                W[4] = k4 - (invx * (FL[4] + FR[4]) + invy * (FD0[4] + FU0[4]));
+            } else {
+                W[3] = R0.H;
+                W[4] = 0.0;
+            }
+            if (a.k == 1) {
+                const int nf = SED ? 5 : 3;
+                for (int f = 0; f < nf; f++) W[f] = 0.5 * v0[f] + 0.5 * W[f];
+            }
+            state_filter<SED>(W[0], W[1], W[2], W[4]);
+            out[pos] = (S)W[0];
+            out[nv + pos] = (S)W[1];
+            out[2 * nv + pos] = (S)W[2];
+            if (SED) {
+                out[3 * nv + pos] = (S)W[3];
+                out[4 * nv + pos] = (S)W[4];
+            }
+            if (a.last) {
+                // the next dt is evaluated on the stored state
+                double h = (double)(S)W[0], qx = (double)(S)W[1], qy = (double)(S)W[2];
+                dtmin = fmin(dtmin, cell_dt(P, h, qx, qy));
+                if (a.ext) {
+                    double Hs = (double)(S)W[3];
+                    double* hmax = a.ext;
+                    double* bmax = a.ext + nv;
+                    double* emax = a.ext + 2 * nv;
+                    if (h > P.epsilon_h) {
+                        hmax[pos] = fmax(hmax[pos], h);
+                        emax[pos] = fmax(emax[pos], h - Hs);
+                    }
+                    bmax[pos] = fmax(bmax[pos], Hs);
+                }
+            }
+        }
+
+        // slide the window
+        R0 = R1;
+        R1 = R2;
+        for (int f = 0; f < 5; f++) {
+            U0[f] = U1[f];
+            FD0[f] = FD1[f];
+        }
+        detay0 = detay1;
+        fx0 = fx1;
+    }
+
+    if (a.last) {
+        double m = wave_min(dtmin);
+        if (lane == 0 && m < __longlong_as_double((long long)kInfBits))
+            atomicMin(&C->dt_bits[(step + 1) & 1], (unsigned long long)__double_as_longlong(m));
+    }
+}
+
+// GPU/main_TSUNER.cu:147-148: dt of the initial state into dt_bits[step & 1].
+template <typename S>
+__global__ __launch_bounds__(256) void init_dt_kernel(const S* V, DevParams P, Ctrl* C)
+{
+    const long long nv = P.nvol;
+    double m = __longlong_as_double((long long)kInfBits);
+    for (long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x; i < nv; i += (long long)gridDim.x * blockDim.x)
+        m = fmin(m, cell_dt(P, (double)V[i], (double)V[nv + i], (double)V[2 * nv + i]));
+    m = wave_min(m);
+    if ((threadIdx.x & 63) == 0 && m < __longlong_as_double((long long)kInfBits))
+        atomicMin(&C->dt_bits[C->step & 1], (unsigned long long)__double_as_longlong(m));
+}
+
+// GPU/main_TSUNER.cu:185: t += dt after the stages; the dt slot of the
+// finished step is re-armed for step + 2.
+__global__ void advance_kernel(Ctrl* C, DevParams P)
+{
+    if (threadIdx.x != 0 || blockIdx.x != 0) return;
+    const double t = C->t;
+    if (!(t < C->stop_time)) return;
+    const long long s = C->step;
+    const double dt = finalize_dt(P, C->dt_bits[s & 1]);
+    C->t = t + dt;
+    C->dt_bits[s & 1] = kInfBits;
+    C->step = s + 1;
+}
+
+__global__ void warmup_kernel() {}
+
+__global__ void fill_kernel(double* p, long long n, double v)
+{
+    for (long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x; i < n; i += (long long)gridDim.x * blockDim.x)
+        p[i] = v;
+}
+
+// Deterministic two-pass FP64 sums of h and of (H + hc).
+template <typename S>
+__global__ __launch_bounds__(256) void mass_partial_kernel(const S* V, long long nv, double* partial)
+{
+    __shared__ double sh[2][4];
+    double a = 0.0, b = 0.0;
+    for (long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x; i < nv; i += (long long)gridDim.x * blockDim.x) {
+        a += (double)V[i];
+        b += (double)V[3 * nv + i] + (double)V[4 * nv + i];
+    }
+    for (int o = 32; o > 0; o >>= 1) {
+        a += __shfl_xor(a, o, 64);
+        b += __shfl_xor(b, o, 64);
+    }
+    const int w = threadIdx.x >> 6;
+    if ((threadIdx.x & 63) == 0) {
+        sh[0][w] = a;
+        sh[1][w] = b;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        partial[2 * blockIdx.x] = ((sh[0][0] + sh[0][1]) + sh[0][2]) + sh[0][3];
+        partial[2 * blockIdx.x + 1] = ((sh[1][0] + sh[1][1]) + sh[1][2]) + sh[1][3];
+    }
+}
+
+__global__ void mass_final_kernel(const double* partial, int nb, double* out)
+{
+    if (threadIdx.x != 0 || blockIdx.x != 0) return;
+    double a = 0.0, b = 0.0;
+    for (int i = 0; i < nb; i++) {
+        a += partial[2 * i];
+        b += partial[2 * i + 1];
+    }
+    out[0] = a;
+    out[1] = b;
+}
+
+__global__ void kat_edge_kernel(DevParams P, int n, const double* left, const double* right, const int* kind,
+                                const int* fixed, double* Dm, double* Dp)
+{
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    double L[5], R[5], dm[5], dp[5];
+    for (int f = 0; f < 5; f++) {
+        L[f] = left[5 * i + f];
+        R[f] = right[5 * i + f];
+    }
+    edge_flux<true>(P, kind[i], fixed[i] != 0 && P.bedload, L, R, dm, dp);
+    for (int f = 0; f < 5; f++) {
+        Dm[5 * i + f] = dm[f];
+        Dp[5 * i + f] = dp[f];
+    }
+}
+
+// what: 0 limiter(a, b); 1 velocity(a=h, b=qx, c=qy); 2 state filter
+// (a, b, c, d) = (h, qx, qy, hc); 3 cell dt (a, b, c); 4 erosion rate
+// (h, qx, qy, hc); 5 bedload (h, un, ut).
+__global__ void kat_point_kernel(int what, int law, int n, const double* a, const double* b, const double* c,
+                                 const double* d, double* o1, double* o2, double* o3, double* o4, DevParams P)
+{
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    switch (what) {
+    case 0:
+        o1[i] = limiter(law, a[i], b[i]);
+        break;
+    case 1: {
+        Vel u = velocity(a[i], b[i], c[i]);
+        o1[i] = u.x;
+        o2[i] = u.y;
+        o3[i] = u.m;
+        break;
+    }
+    case 2: {
+        double h = a[i], qx = b[i], qy = c[i], hc = d[i];
+        state_filter<true>(h, qx, qy, hc);
+        o1[i] = h;
+        o2[i] = qx;
+        o3[i] = qy;
+        o4[i] = hc;
+        break;
+    }
+    case 3:
+        o1[i] = cell_dt(P, a[i], b[i], c[i]);
+        break;
+    case 4: {
+        Vel u = velocity(a[i], b[i], c[i]);
+        o1[i] = erosion_rate(P, a[i], u, d[i]);
+        break;
+    }
+    case 5: {
+        Vel u = velocity(1.0, b[i], c[i]);
+        u.x = b[i];
+        u.y = c[i];
+        u.m = sqrt(u.x * u.x + u.y * u.y);
+        o1[i] = bedload(P, a[i], u);
+        break;
+    }
+    }
+}
+
+}  // namespace
+
+hipError_t launch_stage(const StageArgs& a, const DevParams& P, bool sed, int precision, hipStream_t s)
+{
+    dim3 grid(a.ntiles), block(64);
+    if (precision == 0) {
+        if (sed)
+            hipLaunchKernelGGL((stage_kernel<double, true>), grid, block, 0, s, a, P);
+        else
+            hipLaunchKernelGGL((stage_kernel<double, false>), grid, block, 0, s, a, P);
+    } else {
+        if (sed)
+            hipLaunchKernelGGL((stage_kernel<float, true>), grid, block, 0, s, a, P);
+        else
+            hipLaunchKernelGGL((stage_kernel<float, false>), grid, block, 0, s, a, P);
+    }
+    return hipGetLastError();
+}
+
+hipError_t launch_init_dt(const void* V, const DevParams& P, Ctrl* ctrl, int precision, hipStream_t s)
+{
+    long long nb = (P.nvol + 255) / 256;
+    int grid = (int)(nb < 4096 ? (nb > 0 ? nb : 1) : 4096);
+    if (precision == 0)
+        hipLaunchKernelGGL(init_dt_kernel<double>, dim3(grid), dim3(256), 0, s, (const double*)V, P, ctrl);
+    else
+        hipLaunchKernelGGL(init_dt_kernel<float>, dim3(grid), dim3(256), 0, s, (const float*)V, P, ctrl);
+    return hipGetLastError();
+}
+
+hipError_t launch_warmup(hipStream_t s)
+{
+    hipLaunchKernelGGL(warmup_kernel, dim3(2, 2), dim3(16, 16), 0, s);
+    return hipGetLastError();
+}
+
+hipError_t launch_advance(Ctrl* ctrl, const DevParams& P, hipStream_t s)
+{
+    hipLaunchKernelGGL(advance_kernel, dim3(1), dim3(64), 0, s, ctrl, P);
+    return hipGetLastError();
+}
+
+hipError_t launch_fill(double* p, long long n, double v, hipStream_t s)
+{
+    long long nb = (n + 255) / 256;
+    int grid = (int)(nb < 4096 ? (nb > 0 ? nb : 1) : 4096);
+    hipLaunchKernelGGL(fill_kernel, dim3(grid), dim3(256), 0, s, p, n, v);
+    return hipGetLastError();
+}
+
+int mass_blocks(long long nvol)
+{
+    long long nb = (nvol + 255) / 256;
+    return (int)(nb < 1024 ? (nb > 0 ? nb : 1) : 1024);
+}
+
+hipError_t launch_mass(const void* V, long long nvol, int precision, double* partial, int nblocks, double* out,
+                       hipStream_t s)
+{
+    if (precision == 0)
+        hipLaunchKernelGGL(mass_partial_kernel<double>, dim3(nblocks), dim3(256), 0, s, (const double*)V, nvol, partial);
+    else
+        hipLaunchKernelGGL(mass_partial_kernel<float>, dim3(nblocks), dim3(256), 0, s, (const float*)V, nvol, partial);
+    hipLaunchKernelGGL(mass_final_kernel, dim3(1), dim3(64), 0, s, partial, nblocks, out);
+    return hipGetLastError();
+}
+
+hipError_t launch_kat_edge(const DevParams& P, int n, const double* left, const double* right, const int* kind,
+                           const int* fixed, double* Dm, double* Dp, hipStream_t s)
+{
+    hipLaunchKernelGGL(kat_edge_kernel, dim3((n + 63) / 64), dim3(64), 0, s, P, n, left, right, kind, fixed, Dm, Dp);
+    return hipGetLastError();
+}
+
+hipError_t launch_kat_point(int what, int law, int n, const double* a, const double* b, const double* c,
+                            const double* d, double* o1, double* o2, double* o3, double* o4, const DevParams& P,
+                            hipStream_t s)
+{
+    hipLaunchKernelGGL(kat_point_kernel, dim3((n + 63) / 64), dim3(64), 0, s, what, law, n, a, b, c, d, o1, o2, o3,
+                       o4, P);
+    return hipGetLastError();
+}
+
+}  // namespace tsu

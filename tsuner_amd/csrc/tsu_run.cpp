@@ -1,0 +1,286 @@
+// tsu_run.cpp -- whole-run entries: TSUNER() (the reference's C ABI) and
+// tsu_run().  The time loop follows GPU/main_TSUNER.cu:146-212: initial dt,
+// `while (t < total_time)` with a NetCDF snapshot whenever t has reached the
+// next save time, final snapshot and the running extrema at the end.  Steps
+// are enqueued in batches with dt and t kept on the device; a batch stops
+// itself on the device at the next save time (kernels are no-ops once
+// t >= stop time), so snapshots land on the same steps as in the reference.
+#define TSU_BUILDING_LIBRARY 1
+#include "../../include/tsuner.h"
+
+#include <chrono>
+#include <cmath>
+#include <cstdio>
+#include <ctime>
+#include <string>
+#include <vector>
+
+#include "tsu_netcdf.h"
+
+namespace {
+
+constexpr double kEpsilon = 2.2204460492503131e-16;
+
+// GPU/netcdf.cu:38-242: dimensions, variables and attributes of the output.
+struct Output {
+    tsu::NcWriter nc;
+    int vb = -1, vhmax = -1, vbmin = -1, veta_max = -1, vtime = -1, veta = -1, vux = -1, vuy = -1, vc = -1;
+    int npx = 0, npy = 0;
+
+    bool open(const std::string& path, const tsu_sim_params& p)
+    {
+        npx = p.npx;
+        npy = p.npy;
+        int dx = nc.add_dim("x", (size_t)npx);
+        int dy = nc.add_dim("y", (size_t)npy);
+        int dt = nc.add_dim("time", 0);
+        int vx = nc.add_var("x", {dx});
+        int vy = nc.add_var("y", {dy});
+        vb = nc.add_var("b", {dt, dy, dx});
+        vhmax = nc.add_var("hmax", {dy, dx});
+        vbmin = nc.add_var("bmin", {dy, dx});
+        veta_max = nc.add_var("eta_max", {dy, dx});
+        vtime = nc.add_var("time", {dt});
+        veta = nc.add_var("eta", {dt, dy, dx});
+        nc.att_text(veta, "units", "meters");
+        nc.att_text(veta, "long_name", "Depth");
+        vux = nc.add_var("ux", {dt, dy, dx});
+        nc.att_text(vux, "units", "meters/second");
+        nc.att_text(vux, "long_name", "x-velocity");
+        vuy = nc.add_var("uy", {dt, dy, dx});
+        nc.att_text(vuy, "units", "meters/second");
+        nc.att_text(vuy, "long_name", "y-velocity");
+        vc = nc.add_var("c", {dt, dy, dx});
+        nc.att_text(vc, "units", "adimensional");
+        nc.att_text(vc, "long_name", "Volumetric Sediment Concentration");
+        const double fill = -1e+30;
+        nc.att_double(vc, "missing_value", fill);
+        nc.att_double(vc, "_FillValue", fill);
+        const char* names[3] = {"hmax", "bmin", "eta_max"};
+        int ids[3] = {vhmax, vbmin, veta_max};
+        for (int k = 0; k < 3; k++) {
+            nc.att_text(ids[k], "long_name", names[k]);
+            nc.att_text(ids[k], "standard_name", names[k]);
+            nc.att_text(ids[k], "units", "meters");
+            nc.att_double(ids[k], "missing_value", fill);
+            nc.att_double(ids[k], "_FillValue", fill);
+        }
+        nc.att_text(vx, "long_name", "x dim");
+        nc.att_text(vx, "units", "meters");
+        nc.att_text(vy, "long_name", "y dim");
+        nc.att_text(vy, "units", "meters");
+        nc.att_text(vtime, "long_name", "Time");
+        nc.att_text(vtime, "units", "seconds");
+        for (int v : {veta, vux, vuy}) {
+            nc.att_double(v, "missing_value", fill);
+            nc.att_double(v, "_FillValue", fill);
+        }
+        nc.att_text(-1, "Conventions", "CF-1.0");
+        nc.att_text(-1, "title", "TSUNER model output (MI355X solver)");
+        char when[32];
+        std::time_t now = std::time(nullptr);
+        std::strftime(when, sizeof(when), "%Y-%m-%d %H:%M:%S", std::localtime(&now));
+        nc.att_text(-1, "history", when);
+        nc.att_double(-1, "simulation_time", p.total_time);
+        nc.att_double(-1, "Sediment_relative_density", p.R);
+        nc.att_double(-1, "Sediment_mean_diameter", p.DS);
+        nc.att_double(-1, "Friction", p.FR * 8.0);
+        nc.att_double(-1, "Sediment_porosity", p.porosity);
+        nc.att_double(-1, "Sediment_critical_theta", p.OC);
+        nc.att_double(-1, "Geometric_mean_size_of_suspended_sediment", p.Dsg);
+        if (!nc.create(path)) return false;
+        // GPU/netcdf.cu:256-268: cell-centre coordinates
+        double ddx = (p.xmax - p.xmin) / npx, ddy = (p.ymax - p.ymin) / npy;
+        std::vector<double> xs((size_t)npx), ys((size_t)npy);
+        for (int i = 0; i < npx; i++) xs[(size_t)i] = p.xmin + (i + 0.5) * ddx;
+        for (int j = 0; j < npy; j++) ys[(size_t)j] = p.ymin + (j + 0.5) * ddy;
+        return nc.put_var(vx, xs.data()) && nc.put_var(vy, ys.data());
+    }
+
+    // GPU/netcdf.cu:379-435: eta = h - H, ux = qx/(h+eps), uy, b = -H, c = hc/(h+eps)
+    bool snapshot(size_t rec, double t, const double* V)
+    {
+        const size_t nv = (size_t)npx * npy;
+        std::vector<double> w(nv);
+        bool good = nc.put_record(vtime, rec, &t);
+        for (size_t i = 0; i < nv; i++) w[i] = V[i] - V[3 * nv + i];
+        good = good && nc.put_record(veta, rec, w.data());
+        for (size_t i = 0; i < nv; i++) w[i] = V[nv + i] / (V[i] + kEpsilon);
+        good = good && nc.put_record(vux, rec, w.data());
+        for (size_t i = 0; i < nv; i++) w[i] = V[2 * nv + i] / (V[i] + kEpsilon);
+        good = good && nc.put_record(vuy, rec, w.data());
+        for (size_t i = 0; i < nv; i++) w[i] = -V[3 * nv + i];
+        good = good && nc.put_record(vb, rec, w.data());
+        for (size_t i = 0; i < nv; i++) w[i] = V[4 * nv + i] / (V[i] + kEpsilon);
+        good = good && nc.put_record(vc, rec, w.data());
+        return good && nc.sync();
+    }
+};
+
+struct StateGuard {
+    tsu_state* st = nullptr;
+    ~StateGuard()
+    {
+        if (st) tsu_state_destroy(st);
+    }
+};
+
+}  // namespace
+
+extern "C" {
+
+TSU_API int tsu_run(const double* V, const int* fixH, const tsu_sim_params* params, const tsu_scheme* scheme,
+                    int precision, const char* output_prefix, const double* CC, const double* tCC,
+                    unsigned long timeCC, double* time_gpu, int device)
+{
+    if (!V || !fixH || !params || !scheme) return TSU_ERR_ARG;
+    StateGuard g;
+    int rc = tsu_state_create(&g.st, params, scheme, precision, device);
+    if (rc) return rc;
+    tsu_state* st = g.st;
+    rc = tsu_state_upload(st, V, fixH);
+    if (rc) return rc;
+    bool inflow = false;
+    for (int i = 0; i < 4; i++) inflow = inflow || scheme->bc[i] == TSU_BC_INFLOW;
+    if (inflow) {
+        rc = tsu_state_set_inflow(st, CC, tCC, timeCC);
+        if (rc) return rc;
+    }
+    const size_t nv = (size_t)params->npx * params->npy;
+    std::vector<double> host;
+    try {
+        host.resize(5 * nv);
+    } catch (...) {
+        return TSU_ERR_HOST;
+    }
+    Output out;
+    const bool write = output_prefix && output_prefix[0];
+    if (write && !out.open(std::string(output_prefix) + ".nc", *params)) return TSU_ERR_HOST;
+    rc = tsu_init_dt(st);
+    if (rc) return rc;
+    double t = 0.0, dt = 0.0;
+    long long steps = 0;
+    tsu_get_time(st, &t, &dt, &steps);
+    std::fprintf(stdout, "Initial deltaT = %e sec\n", dt);
+
+    auto t0 = std::chrono::steady_clock::now();
+    const double total = params->total_time, save = params->save_time;
+    double next_save = t;
+    size_t rec = 0;
+    std::vector<double> hmax, bmax, emax;
+    if (scheme->extrema) {
+        hmax.resize(nv);
+        bmax.resize(nv);
+        emax.resize(nv);
+    }
+    while (t < total) {
+        if (write && save >= 0.0 && t >= next_save) {
+            if ((rc = tsu_state_download(st, host.data()))) return rc;
+            if (scheme->extrema) {
+                if ((rc = tsu_get_extrema(st, hmax.data(), bmax.data(), emax.data()))) return rc;
+                out.nc.put_var(out.vhmax, hmax.data());
+                out.nc.put_var(out.veta_max, emax.data());
+            }
+            out.snapshot(rec++, t, host.data());
+            next_save += save;
+        }
+        double stop = total;
+        if (write && save > 0.0 && next_save < stop) stop = next_save;
+        long long batch = 1;
+        if (!(write && save == 0.0) && dt > 0.0) {
+            double est = std::ceil((stop - t) / dt);
+            batch = est < 1.0 ? 1 : (est > 512.0 ? 512 : (long long)est);
+        }
+        if ((rc = tsu_set_stop_time(st, stop))) return rc;
+        if ((rc = tsu_step(st, (int)batch))) return rc;
+        if ((rc = tsu_get_time(st, &t, &dt, &steps))) return rc;
+        std::fprintf(stdout, "Iteration %3lld, deltaT = %e sec, Time = %g sec\n", steps, dt, t);
+    }
+    if (write) {
+        if ((rc = tsu_state_download(st, host.data()))) return rc;
+        out.snapshot(rec++, t, host.data());
+        if (scheme->extrema) {
+            if ((rc = tsu_get_extrema(st, hmax.data(), bmax.data(), emax.data()))) return rc;
+            out.nc.put_var(out.vhmax, hmax.data());
+            out.nc.put_var(out.veta_max, emax.data());
+            for (size_t i = 0; i < nv; i++) bmax[i] = -bmax[i];   // GPU/main_TSUNER.cu:207-209
+            out.nc.put_var(out.vbmin, bmax.data());
+        }
+        if (!out.nc.close()) return TSU_ERR_HOST;
+    }
+    auto t1 = std::chrono::steady_clock::now();
+    if (time_gpu) *time_gpu = std::chrono::duration<double>(t1 - t0).count();
+    return TSU_OK;
+}
+
+TSU_API int tsu_write_output(const char* output_prefix, const tsu_sim_params* params, int nrec,
+                             const double* times, const double* V_records, const double* hmax, const double* bmax,
+                             const double* eta_max)
+{
+    if (!output_prefix || !params || nrec < 0 || (nrec > 0 && (!times || !V_records))) return TSU_ERR_ARG;
+    if (params->npx < 1 || params->npy < 1) return TSU_ERR_ARG;
+    Output out;
+    if (!out.open(std::string(output_prefix) + ".nc", *params)) return TSU_ERR_HOST;
+    const size_t nv = (size_t)params->npx * params->npy;
+    for (int r = 0; r < nrec; r++)
+        if (!out.snapshot((size_t)r, times[r], V_records + (size_t)r * 5 * nv)) return TSU_ERR_HOST;
+    if (hmax) out.nc.put_var(out.vhmax, hmax);
+    if (eta_max) out.nc.put_var(out.veta_max, eta_max);
+    if (bmax) {
+        std::vector<double> b(nv);
+        for (size_t i = 0; i < nv; i++) b[i] = -bmax[i];
+        out.nc.put_var(out.vbmin, b.data());
+    }
+    return out.nc.close() ? TSU_OK : TSU_ERR_HOST;
+}
+
+TSU_API int TSUNER(double* dVh, tsu_double2* dVhu, double* dVH, int* fixH, tsu_sim_params params,
+                   char* input_prefix, double* d, double* CC, double* tCC, unsigned long timeCC, double* dyrup,
+                   float* time_dyrup, double* time_gpu, int device, double* tides_bd, float* tides_time,
+                   int tidesNPT)
+{
+    (void)d;
+    (void)dyrup;
+    (void)time_dyrup;
+    (void)tides_bd;
+    (void)tides_time;
+    (void)tidesNPT;
+    if (!dVh || !dVhu || !dVH || !fixH) return TSU_ERR_ARG;
+    const size_t nv = (size_t)params.npx * params.npy;
+    std::vector<double> V;
+    try {
+        V.resize(5 * nv);
+    } catch (...) {
+        return TSU_ERR_HOST;
+    }
+    // GPU/main_TSUNER.cu:104-111: AoS momentum to planar SoA, hc = 0
+    for (size_t i = 0; i < nv; i++) {
+        V[i] = dVh[i];
+        V[nv + i] = dVhu[i].x;
+        V[2 * nv + i] = dVhu[i].y;
+        V[3 * nv + i] = dVH[i];
+        V[4 * nv + i] = 0.0;
+    }
+    // Sim_parameters.hxx:45-54 defaults: order 2, RK, bedload and suspension
+    // on, walls, inflow series when present (Read_files = 1).
+    tsu_scheme s{};
+    s.order = 2;
+    s.bedload = 1;
+    s.suspension = 1;
+    const bool inflow = CC && tCC && timeCC >= 2;
+    for (int i = 0; i < 4; i++) s.bc[i] = inflow ? TSU_BC_INFLOW : TSU_BC_WALL;
+    s.limiter = TSU_LIM_VANLEER;
+    s.bedload_law = TSU_BEDLOAD_ASHIDA_MICHIUE;
+    s.friction_law = TSU_FRICTION_QUADRATIC;
+    s.dt_narrow = 1;
+    s.extrema = 1;
+    s.grass_ag = 0.0;
+    s.manning_n = 0.0;
+    int rc = tsu_run(V.data(), fixH, &params, &s, TSU_FP64, input_prefix, CC, tCC, timeCC, time_gpu, device);
+    // Map to the reference's codes (GPU/main_TSUNER.cu:93-96,124-140).
+    if (rc == TSU_OK) return 0;
+    if (rc == TSU_ERR_DEVICE_MEMORY) return 1;
+    return 2;
+}
+
+}  // extern "C"
