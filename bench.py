@@ -122,6 +122,8 @@ def main():
     ap.add_argument("--tile-rows", type=int, default=0)
     ap.add_argument("--no-fp32", action="store_true")
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-active", action="store_true",
+                    help="skip the active-flow variant (every wet cell moving, sediment suspended)")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
         print("bench.py measures one GPU only: this solver path runs on a single MI355X and no multi-GPU "
@@ -136,6 +138,21 @@ def main():
 
     r64 = run_variant(sc, "fp64", args.steps, args.warmup, args.tile_rows)
     r32 = None if args.no_fp32 else run_variant(sc, "fp32", args.steps, args.warmup, args.tile_rows)
+    active = None
+    if not args.no_active:
+        # Same domain with flow everywhere: the configured run is mostly still
+        # water during its first 1000 steps, this one exercises every branch.
+        sa = S.Scenario(sc.name + "_active", sc.params, sc.scheme, sc.V.copy(), sc.fixH, sc.CC, sc.tCC)
+        sa.V[1] = sa.V[0] * 1.0
+        sa.V[2] = sa.V[0] * 0.25
+        sa.V[4] = sa.V[0] * 1e-3
+        a64 = run_variant(sa, "fp64", args.steps, args.warmup, args.tile_rows)
+        a32 = None if args.no_fp32 else run_variant(sa, "fp32", args.steps, args.warmup, args.tile_rows)
+        active = {"fp64": a64["value"], "fp64_roofline_frac": a64["roofline"]["frac"],
+                  "fp32": None if a32 is None else a32["value"],
+                  "fp32_roofline_frac": None if a32 is None else a32["roofline"]["frac"],
+                  "state": "initial state plus u = 1 m/s, v = 0.25 m/s and c = 1e-3 in every wet cell"}
+        del sa
     cpu = None
     if not args.no_cpu:
         log("[bench] cpu baseline")
@@ -165,6 +182,7 @@ def main():
                                           "dtype": "f32 planes (f64 arithmetic)", "bytes_per_cell_update": 100,
                                           "roofline": r32["roofline"]},
         "stage_ms_mean": r64["stage_ms_mean"],
+        "active_flow": active,
     }
     print(json.dumps(out), flush=True)
 

@@ -160,6 +160,10 @@ TSU_API void *tsu_stream(tsu_state *st);
 /* Device pointer of the U^n planes (element size 8 or 4 by precision). */
 TSU_API void *tsu_device_state(tsu_state *st);
 
+/* Achievable HBM bandwidth: times reps launches of a plain device copy of
+ * `bytes` bytes and returns (read + write bytes) / mean launch time in GB/s. */
+TSU_API int tsu_bandwidth_probe(int device, long long bytes, int reps, double *gbs);
+
 TSU_API const char *tsu_last_error(void);
 TSU_API const char *tsu_version(void);
 

@@ -277,8 +277,9 @@ def test_fp32_planes_close_to_fp64(oracle):
     o.step(20)
     g.step(20)
     a, b = o.state(), g.state()
+    qscale = max(np.abs(a[1]).max(), np.abs(a[2]).max())
     for f, name in enumerate(["h", "qx", "qy", "H", "hc"]):
-        scale = max(np.abs(a[f]).max(), 1e-12)
+        scale = qscale if f in (1, 2) else max(np.abs(a[f]).max(), 1e-12)
         err = np.abs(a[f] - b[f]).max() / scale
         say("fp32 rel err %s %.3e" % (name, err))
         assert err < 1e-4, (name, err)

@@ -97,6 +97,7 @@ _PROTOS = {
     "tsu_get_timing": (C.c_int, [C.c_void_p, _dp, C.POINTER(C.c_longlong), C.c_int]),
     "tsu_stream": (C.c_void_p, [C.c_void_p]),
     "tsu_device_state": (C.c_void_p, [C.c_void_p]),
+    "tsu_bandwidth_probe": (C.c_int, [C.c_int, C.c_longlong, C.c_int, _dp]),
     "tsu_last_error": (C.c_char_p, []),
     "tsu_version": (C.c_char_p, []),
     "tsu_kat_edge_flux": (C.c_int, [C.POINTER(SimParams), C.POINTER(Scheme), C.c_int, _dp, _dp, _ip, _ip, _dp,

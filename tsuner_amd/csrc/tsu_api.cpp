@@ -149,6 +149,7 @@ struct tsu_state {
     unsigned long timeCC = 0;
     double* mass_buf = nullptr;
     int mass_nb = 0;
+    DevParams* dP = nullptr;   // device copy of P read by the stage kernels
     hipStream_t stream = nullptr;
     int tile_rows = 0, nstrips = 0, nchunks = 0, ntiles = 0;
     int graph_chunk = 0;
@@ -220,6 +221,7 @@ void free_state(tsu_state* st)
     (void)hipFree(st->CC);
     (void)hipFree(st->tCC);
     (void)hipFree(st->mass_buf);
+    (void)hipFree(st->dP);
     if (st->stream) (void)hipStreamDestroy(st->stream);
     delete st;
 }
@@ -235,6 +237,7 @@ tsu::StageArgs stage_args(const tsu_state* st, int k)
     a.tile_rows = st->tile_rows;
     a.nstrips = st->nstrips;
     a.ntiles = st->ntiles;
+    a.P = st->dP;
     a.k = k;
     if (st->s.order == 1) {
         a.in = st->A;
@@ -273,7 +276,7 @@ int enqueue_step(tsu_state* st)
             st->ev_stage[st->ev_used] = k;
             HIPCHK(hipEventRecord(st->ev[2 * st->ev_used], st->stream));
         }
-        HIPCHK(tsu::launch_stage(a, st->P, st->sed, st->precision, st->stream));
+        HIPCHK(tsu::launch_stage(a, st->sed, st->precision, st->stream));
         if (timed) {
             HIPCHK(hipEventRecord(st->ev[2 * st->ev_used + 1], st->stream));
             st->ev_used++;
@@ -364,6 +367,8 @@ TSU_API int tsu_state_create(tsu_state** out, const tsu_sim_params* p, const tsu
     if (e == hipSuccess) e = hipMalloc((void**)&st->fix, (size_t)st->nvol);
     if (e == hipSuccess) e = hipMalloc((void**)&st->ctrl, sizeof(Ctrl));
     if (e == hipSuccess) e = hipMalloc((void**)&st->mass_buf, (2 * (size_t)st->mass_nb + 2) * sizeof(double));
+    if (e == hipSuccess) e = hipMalloc((void**)&st->dP, sizeof(DevParams));
+    if (e == hipSuccess) e = hipMemcpy(st->dP, &st->P, sizeof(DevParams), hipMemcpyHostToDevice);
     if (e == hipSuccess && s->extrema) e = hipMalloc((void**)&st->ext, 3 * (size_t)st->nvol * sizeof(double));
     if (e != hipSuccess) {
         free_state(st);
@@ -470,6 +475,7 @@ TSU_API int tsu_state_set_inflow(tsu_state* st, const double* CC, const double* 
     st->P.tcc0 = tCC[0];
     st->P.dtcc = tCC[1] - tCC[0];
     st->P.tcc_end = tCC[timeCC - 2];
+    HIPCHK(hipMemcpy(st->dP, &st->P, sizeof(DevParams), hipMemcpyHostToDevice));
     return TSU_OK;
 }
 
@@ -666,6 +672,34 @@ TSU_API int tsu_get_timing(tsu_state* st, double stage_ms[2], long long launches
 TSU_API void* tsu_stream(tsu_state* st) { return st ? (void*)st->stream : nullptr; }
 
 TSU_API void* tsu_device_state(tsu_state* st) { return st ? st->A : nullptr; }
+
+TSU_API int tsu_bandwidth_probe(int device, long long bytes, int reps, double* gbs)
+{
+    if (bytes < 16 || reps < 1 || !gbs) return fail(TSU_ERR_ARG, "bad argument");
+    bytes &= ~15LL;
+    DeviceGuard g(device);
+    void *a = nullptr, *b = nullptr;
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+    hipError_t e = hipMalloc(&a, (size_t)bytes);
+    if (e == hipSuccess) e = hipMalloc(&b, (size_t)bytes);
+    if (e == hipSuccess) e = hipMemset(a, 0, (size_t)bytes);
+    if (e == hipSuccess) e = hipEventCreate(&e0);
+    if (e == hipSuccess) e = hipEventCreate(&e1);
+    if (e == hipSuccess) e = tsu::launch_copy(a, b, bytes, nullptr);   // warm-up
+    if (e == hipSuccess) e = hipEventRecord(e0, nullptr);
+    for (int i = 0; i < reps && e == hipSuccess; i++) e = tsu::launch_copy(a, b, bytes, nullptr);
+    if (e == hipSuccess) e = hipEventRecord(e1, nullptr);
+    if (e == hipSuccess) e = hipEventSynchronize(e1);
+    float ms = 0.0f;
+    if (e == hipSuccess) e = hipEventElapsedTime(&ms, e0, e1);
+    if (e0) (void)hipEventDestroy(e0);
+    if (e1) (void)hipEventDestroy(e1);
+    (void)hipFree(a);
+    (void)hipFree(b);
+    if (e != hipSuccess) return fail(TSU_ERR_HIP, std::string("bandwidth probe: ") + hipGetErrorString(e));
+    *gbs = 2.0 * (double)bytes * reps / (ms * 1e-3) / 1e9;
+    return TSU_OK;
+}
 
 TSU_API int tsu_kat_edge_flux(const tsu_sim_params* p, const tsu_scheme* s, int n, const double* left,
                               const double* right, const int* normal, const int* fixed, double* Dm, double* Dp,

@@ -72,12 +72,18 @@ struct Ctrl {
 
 __device__ __forceinline__ int sgn(double x) { return (fabs(x) < kEpsilon) ? 0 : ((x > 0) ? 1 : -1); }
 
-// Reconstructor.cu:19-31
+// Reconstructor.cu:19-31.  When a and b do not share a strict sign the
+// numerator a|b| + b|a| is exactly zero (for finite products), so the
+// division is only evaluated for same-sign slopes.
 __device__ __forceinline__ double lim_vanleer(double a, double b)
 {
     double aa = fabs(a), bb = fabs(b);
     double s = aa + bb;
-    return (s > kEpsilon) ? (a * bb + b * aa) / s : 0.0;
+    double d = 0.0;
+    if ((a > 0.0 && b > 0.0) || (a < 0.0 && b < 0.0)) {
+        if (s > kEpsilon) d = (a * bb + b * aa) / s;
+    }
+    return d;
 }
 
 __device__ __forceinline__ double lim_minmod(double a, double b)
@@ -92,32 +98,39 @@ __device__ __forceinline__ double limiter(int law, double a, double b)
     return law == LIM_MINMOD ? lim_minmod(a, b) : lim_vanleer(a, b);
 }
 
+// sqrt is monotone and correctly rounded, so sqrt(s) > 10 exactly when
+// s > nextafter(100, +inf).
+constexpr double kVmax2Cut = 100.00000000000001421085471520200371742248535156250;
+
 struct Vel {
     double x, y;
-    double m;   // |u| of the returned (possibly capped) velocity
+    double s;   // u.x*u.x + u.y*u.y of the returned (possibly capped) velocity
 };
 
-// SedFlux.cuh:26-49.  m equals the reference's later sqrt(u.x*u.x+u.y*u.y)
-// (Reconstructor.cu:43,74; SedFlux.cuh:157): the same expression on the same
-// operands when the cap is inactive, recomputed when it is active.
+// |u| as the reference evaluates it after u_calculator (Reconstructor.cu:43,74;
+// SedFlux.cuh:157): sqrt(u.x*u.x + u.y*u.y).
+__device__ __forceinline__ double speed(const Vel& u) { return sqrt(u.s); }
+
+// SedFlux.cuh:26-49.  Divisions with a zero numerator are skipped (0/d is
+// 0); the cap test uses s instead of sqrt(s) (see kVmax2Cut).
 __device__ __forceinline__ Vel velocity(double h, double qx, double qy)
 {
     Vel u;
+    u.x = 0.0;
+    u.y = 0.0;
     if (h >= kEps) {
         double d = h + kEpsilon;
-        u.x = qx / d;
-        u.y = qy / d;
-    } else {
-        u.x = 0.0;
-        u.y = 0.0;
+        if (qx != 0.0) u.x = qx / d;
+        if (qy != 0.0) u.y = qy / d;
     }
-    double m = sqrt(u.x * u.x + u.y * u.y);
-    if (m > kVmax) {
+    double s = u.x * u.x + u.y * u.y;
+    if (s > kVmax2Cut) {
+        double m = sqrt(s);
         u.x = u.x / m * kVmax;
         u.y = u.y / m * kVmax;
-        m = sqrt(u.x * u.x + u.y * u.y);
+        s = u.x * u.x + u.y * u.y;
     }
-    u.m = m;
+    u.s = s;
     return u;
 }
 
@@ -128,7 +141,7 @@ __device__ __forceinline__ void state_filter(double& h, double& qx, double& qy, 
     h = (h > 0.0) ? h : 0.0;
     double c = 0.0;
     if (SED) {
-        c = hc / (h + kEpsilon);
+        if (hc != 0.0) c = hc / (h + kEpsilon);
         c = (c > 0.0) ? c : 0.0;
     }
     Vel u = velocity(h, qx, qy);
@@ -142,7 +155,7 @@ __device__ __forceinline__ void state_filter(double& h, double& qx, double& qy, 
 __device__ __forceinline__ double bedload(const DevParams& P, double h, const Vel& u)
 {
     if (!(h > P.HQB && h > P.fiveDS)) return 0.0;
-    double m = u.m;
+    double m = speed(u);
     if (P.bedload_law == BEDLOAD_GRASS) return -(P.grass_k * u.x * m * m);
     double O = P.Opar * m * m;
     if (!(O > P.OC)) return 0.0;
@@ -185,16 +198,23 @@ __device__ __forceinline__ void riemann(const DevParams& P, bool bed_mobile,
                                         double hr, double zr, double qnr, double qtr, double hcr,
                                         double* Dm, double* Dp)
 {
+    double zmH = fmin(zl, zr);
+    double h0 = fmax(hl - zl + zmH, 0.0);
+    double h1 = fmax(hr - zr + zmH, 0.0);
+    if (h0 == 0.0 && h1 == 0.0) {
+        // Both reconstructed depths vanish: every flux, jump and bed term
+        // below is a product with h0, h1, their difference or a zero jump,
+        // so Dm = Dp = 0 (the speeds stay finite: |u| <= VMAX).
+        for (int f = 0; f < (SED ? 5 : 3); f++) Dm[f] = Dp[f] = 0.0;
+        return;
+    }
     Vel u0 = velocity(hl, qnl, qtl);
     Vel u1 = velocity(hr, qnr, qtr);
     double c0 = 0.0, c1 = 0.0;
     if (SED) {
-        c0 = hcl / (hl + kEpsilon);
-        c1 = hcr / (hr + kEpsilon);
+        if (hcl != 0.0) c0 = hcl / (hl + kEpsilon);
+        if (hcr != 0.0) c1 = hcr / (hr + kEpsilon);
     }
-    double zmH = fmin(zl, zr);
-    double h0 = fmax(hl - zl + zmH, 0.0);
-    double h1 = fmax(hr - zr + zmH, 0.0);
     double q0n = u0.x * h0, q0t = u0.y * h0;
     double q1n = u1.x * h1, q1t = u1.y * h1;
     double deta = h1 - h0;
@@ -261,10 +281,11 @@ __device__ __forceinline__ void riemann(const DevParams& P, bool bed_mobile,
 
 // Reconstructor.cu:57-88 with SedFlux.cuh:51-99: exchange rate phi between
 // the bed (H) and the suspended load (hc), evaluated on U^n.
-__device__ __forceinline__ double erosion_rate(const DevParams& P, double h, const Vel& u, double hcv)
+__device__ __forceinline__ double erosion_rate(const DevParams& P, double h, double modu, double hcv)
 {
-    double c = hcv / (h + P.epsilon_h);
-    double Z = P.ka * (u.m * P.sqrtFR) / P.vs;
+    double c = 0.0;
+    if (hcv != 0.0) c = hcv / (h + P.epsilon_h);
+    double Z = P.ka * (modu * P.sqrtFR) / P.vs;
     double Z5 = Z * Z * Z * Z * Z;
     double Es = (1.3e-7 * Z5) / (1 + 4.3e-7 * Z5);
     double Fe = P.vs1p * Es;

@@ -26,12 +26,14 @@ struct StageArgs {
     int tile_rows;
     int nstrips;
     int ntiles;
+    const DevParams* P;   // run constants in device memory
 };
 
 // precision: 0 = FP64 planes, 1 = FP32 planes (FP64 arithmetic).
-hipError_t launch_stage(const StageArgs& a, const DevParams& P, bool sed, int precision, hipStream_t s);
+hipError_t launch_stage(const StageArgs& a, bool sed, int precision, hipStream_t s);
 hipError_t launch_init_dt(const void* V, const DevParams& P, Ctrl* ctrl, int precision, hipStream_t s);
 hipError_t launch_warmup(hipStream_t s);
+hipError_t launch_copy(const void* src, void* dst, long long bytes, hipStream_t s);
 hipError_t launch_advance(Ctrl* ctrl, const DevParams& P, hipStream_t s);
 hipError_t launch_fill(double* p, long long n, double v, hipStream_t s);
 // Sums of h and of (H + hc) in FP64 into out[0], out[1] (device pointer),

@@ -57,7 +57,7 @@ __device__ __forceinline__ double wave_min(double v)
 
 struct Cell {
     double h, qx, qy, H, hc;   // state
-    double ux, uy, c;          // velocity and concentration
+    double ux, uy, us, c;      // velocity, |u|^2 and concentration
 };
 
 template <typename S, bool SED>
@@ -76,7 +76,9 @@ __device__ __forceinline__ void derive_cell(Cell& v)
     Vel u = velocity(v.h, v.qx, v.qy);
     v.ux = u.x;
     v.uy = u.y;
-    v.c = SED ? v.hc / (v.h + kEpsilon) : 0.0;
+    v.us = u.s;
+    v.c = 0.0;
+    if (SED && v.hc != 0.0) v.c = v.hc / (v.h + kEpsilon);
 }
 
 // Reconstructor.cu:146-194 along one axis: faces on the minus side (Fm) and
@@ -164,8 +166,11 @@ __device__ __forceinline__ void edge_flux(const DevParams& P, int kind, bool mob
 }
 
 template <typename S, bool SED>
-__global__ __launch_bounds__(64) void stage_kernel(StageArgs a, DevParams P)
+__global__ __launch_bounds__(64) void stage_kernel(StageArgs a)
 {
+    // Run constants are read through a pointer: fields used only in rare
+    // branches are then loaded there instead of pinning scalar registers.
+    const DevParams& P = *a.P;
     const int lane = threadIdx.x;
     const int tile = xcd_tile(blockIdx.x, gridDim.x);
     const int strip = tile % a.nstrips, chunk = tile / a.nstrips;
@@ -351,18 +356,26 @@ __global__ __launch_bounds__(64) void stage_kernel(StageArgs a, DevParams P)
                 k1 -= dtaxG * R0.h * detax;
                 k2 -= dtayG * R0.h * detay0;
             }
-            Vel un = velocity(v0[0], v0[1], v0[2]);
+            Vel un;
+            if (a.k == 0) {   // U^n is this stage's input: its velocity is known
+                un.x = R0.ux;
+                un.y = R0.uy;
+                un.s = R0.us;
+            } else {
+                un = velocity(v0[0], v0[1], v0[2]);
+            }
+            const double modu = speed(un);
             double cf;
             if (P.friction_law == FRIC_MANNING) {
                 double hp = v0[0] + kEpsilon;
-                cf = dtGnn * un.m / (hp * cbrt(hp));
+                cf = dtGnn * modu / (hp * cbrt(hp));
             } else {
-                cf = dtFR * un.m / (v0[0] + kEpsilon);
+                cf = dtFR * modu / (v0[0] + kEpsilon);
             }
             k1 /= (1.0 + cf);
             k2 /= (1.0 + cf);
             if (SED && P.suspension) {
-                double phi = erosion_rate(P, v0[0], un, v0[4]);
+                double phi = erosion_rate(P, v0[0], modu, v0[4]);
                 k3 -= dt * phi;
                 k4 += dt * phi;
             }
@@ -454,6 +467,15 @@ __global__ void advance_kernel(Ctrl* C, DevParams P)
 
 __global__ void warmup_kernel() {}
 
+// Plain streaming copy (16 B per lane per access), the reference point for
+// achievable HBM bandwidth and for calibrating byte counters.
+__global__ __launch_bounds__(256) void copy_kernel(const double2* __restrict__ src, double2* __restrict__ dst,
+                                                   long long n)
+{
+    for (long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x; i < n; i += (long long)gridDim.x * blockDim.x)
+        dst[i] = src[i];
+}
+
 __global__ void fill_kernel(double* p, long long n, double v)
 {
     for (long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x; i < n; i += (long long)gridDim.x * blockDim.x)
@@ -531,7 +553,7 @@ __global__ void kat_point_kernel(int what, int law, int n, const double* a, cons
         Vel u = velocity(a[i], b[i], c[i]);
         o1[i] = u.x;
         o2[i] = u.y;
-        o3[i] = u.m;
+        o3[i] = speed(u);
         break;
     }
     case 2: {
@@ -548,14 +570,14 @@ __global__ void kat_point_kernel(int what, int law, int n, const double* a, cons
         break;
     case 4: {
         Vel u = velocity(a[i], b[i], c[i]);
-        o1[i] = erosion_rate(P, a[i], u, d[i]);
+        o1[i] = erosion_rate(P, a[i], speed(u), d[i]);
         break;
     }
     case 5: {
         Vel u = velocity(1.0, b[i], c[i]);
         u.x = b[i];
         u.y = c[i];
-        u.m = sqrt(u.x * u.x + u.y * u.y);
+        u.s = u.x * u.x + u.y * u.y;
         o1[i] = bedload(P, a[i], u);
         break;
     }
@@ -564,19 +586,19 @@ __global__ void kat_point_kernel(int what, int law, int n, const double* a, cons
 
 }  // namespace
 
-hipError_t launch_stage(const StageArgs& a, const DevParams& P, bool sed, int precision, hipStream_t s)
+hipError_t launch_stage(const StageArgs& a, bool sed, int precision, hipStream_t s)
 {
     dim3 grid(a.ntiles), block(64);
     if (precision == 0) {
         if (sed)
-            hipLaunchKernelGGL((stage_kernel<double, true>), grid, block, 0, s, a, P);
+            hipLaunchKernelGGL((stage_kernel<double, true>), grid, block, 0, s, a);
         else
-            hipLaunchKernelGGL((stage_kernel<double, false>), grid, block, 0, s, a, P);
+            hipLaunchKernelGGL((stage_kernel<double, false>), grid, block, 0, s, a);
     } else {
         if (sed)
-            hipLaunchKernelGGL((stage_kernel<float, true>), grid, block, 0, s, a, P);
+            hipLaunchKernelGGL((stage_kernel<float, true>), grid, block, 0, s, a);
         else
-            hipLaunchKernelGGL((stage_kernel<float, false>), grid, block, 0, s, a, P);
+            hipLaunchKernelGGL((stage_kernel<float, false>), grid, block, 0, s, a);
     }
     return hipGetLastError();
 }
@@ -595,6 +617,13 @@ hipError_t launch_init_dt(const void* V, const DevParams& P, Ctrl* ctrl, int pre
 hipError_t launch_warmup(hipStream_t s)
 {
     hipLaunchKernelGGL(warmup_kernel, dim3(2, 2), dim3(16, 16), 0, s);
+    return hipGetLastError();
+}
+
+hipError_t launch_copy(const void* src, void* dst, long long bytes, hipStream_t s)
+{
+    long long n = bytes / 16;
+    hipLaunchKernelGGL(copy_kernel, dim3(8192), dim3(256), 0, s, (const double2*)src, (double2*)dst, n);
     return hipGetLastError();
 }
 
