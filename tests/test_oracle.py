@@ -1,0 +1,361 @@
+"""Pins of the CPU oracle (oracle/tsu_oracle.c) independent of its own code.
+
+The reference ships no tests, fixtures or golden vectors, and cannot be built
+here (it needs nvcc, CUDA headers and libnetcdf), so each oracle function is
+pinned by what the mathematics fixes: textbook closed forms evaluated in a
+different algebraic form, exact symmetries of the scheme, conservation
+identities, special cases with known answers (consistency, lake at rest,
+uniform states) and convergence to the Ritter dam-break solution.
+"""
+import math
+
+import numpy as np
+import pytest
+
+from tsuner_amd import scenarios as S
+
+G = 9.81
+
+
+@pytest.fixture(scope="module")
+def sed():
+    sc = S.sediment_shelf(n=8, dx=10.0, inflow=False)
+    return sc.params, sc.scheme
+
+
+# ---------------------------------------------------------------- limiters --
+
+def test_vanleer_is_the_harmonic_mean(oracle):
+    rng = np.random.default_rng(0)
+    for _ in range(2000):
+        a, b = rng.uniform(1e-3, 10, 2) * rng.choice([-1, 1])
+        got = oracle.limiter(S.LIM_VANLEER, a, b)
+        assert got == pytest.approx(2 * a * b / (a + b), rel=1e-14)
+        assert abs(got) <= 2 * min(abs(a), abs(b)) * (1 + 1e-15)   # TVD bound
+        assert oracle.limiter(S.LIM_VANLEER, b, a) == got
+        assert oracle.limiter(S.LIM_VANLEER, -a, -b) == -got
+        assert oracle.limiter(S.LIM_VANLEER, a, -b) == 0.0
+    assert oracle.limiter(S.LIM_VANLEER, 1e-17, 1e-17) == 0.0   # |a|+|b| <= DBL_EPSILON
+    assert oracle.limiter(S.LIM_VANLEER, 0.0, 3.0) == 0.0
+
+
+def test_minmod(oracle):
+    assert oracle.limiter(S.LIM_MINMOD, 1.0, 3.0) == 1.0
+    assert oracle.limiter(S.LIM_MINMOD, -4.0, -2.0) == -2.0
+    assert oracle.limiter(S.LIM_MINMOD, -1.0, 2.0) == 0.0
+    assert oracle.limiter(S.LIM_MINMOD, 0.0, 2.0) == 0.0
+
+
+# ------------------------------------------------------- velocity / filter --
+
+def test_velocity_desingularisation_and_cap(oracle):
+    ux, uy = oracle.velocity(2.0, 3.0, -1.0)
+    assert ux == pytest.approx(1.5, rel=1e-15) and uy == pytest.approx(-0.5, rel=1e-15)
+    assert oracle.velocity(9.99e-7, 5.0, 5.0) == (0.0, 0.0)      # h < 1e-6: dry
+    ux, uy = oracle.velocity(1e-6, 1e-6, 0.0)                     # h == 1e-6 counts as wet
+    assert ux == pytest.approx(1.0, rel=1e-9)
+    ux, uy = oracle.velocity(1.0, 30.0, 40.0)                     # |u| = 50 -> capped at 10
+    assert math.hypot(ux, uy) == pytest.approx(10.0, rel=1e-15)
+    assert ux / uy == pytest.approx(0.75, rel=1e-15)
+
+
+def test_state_filter(oracle):
+    h, qx, qy, hc = oracle.state_filter(-0.3, 1.0, 1.0, 0.2)
+    assert (h, qx, qy, hc) == (0.0, 0.0, 0.0, 0.0)
+    h, qx, qy, hc = oracle.state_filter(2.0, 4.0, -2.0, -0.1)     # negative concentration clipped
+    assert h == 2.0 and hc == 0.0
+    assert qx == pytest.approx(4.0, rel=1e-15) and qy == pytest.approx(-2.0, rel=1e-15)
+    h, qx, qy, hc = oracle.state_filter(1.0, 50.0, 0.0, 0.05)     # momentum of a capped velocity
+    assert qx == pytest.approx(10.0, rel=1e-15) and hc == pytest.approx(0.05, rel=1e-15)
+
+
+# ---------------------------------------------------------- sediment laws --
+
+def zhang_settling(d, R, nu=1.002e-6):
+    """Zhang (1961) settling velocity, written as the root of
+    w^2 + 2*(13.95 nu/d) w - 1.09 (R-1) g d = 0."""
+    a = 13.95 * nu / d
+    c = 1.09 * (R - 1) * G * d
+    return c / (a + math.sqrt(a * a + c))   # the positive root, cancellation-free form
+
+
+def test_bedload_ashida_michiue_textbook(oracle, sed):
+    p, s = sed
+    R, d, poro, thc = p["R"], p["DS"], p["porosity"], p["OC"]
+    cf = 0.025 / 8.0                   # bed shear stress tau/rho = cf |u|^2
+    for h, un, ut in [(0.5, 1.2, 0.3), (2.0, -0.8, 0.9), (0.01, 2.0, 0.0), (1.0, 0.1, 0.0)]:
+        u = math.hypot(un, ut)
+        theta = cf * u * u / ((R - 1) * G * d)
+        qstar = 17.0 * max(theta - thc, 0) * max(math.sqrt(theta) - math.sqrt(thc), 0)
+        qb = qstar * math.sqrt((R - 1) * G * d ** 3) / (1 - poro) * un / u
+        got = oracle.bedload_flux(p, s, h, un, ut)
+        assert got == pytest.approx(-qb, rel=1e-12, abs=1e-300)
+    # depth gates h > 5 d and h > HQB
+    assert oracle.bedload_flux(p, s, 5 * d * 0.99, 2.0, 0.0) == 0.0
+
+
+def test_bedload_grass(oracle, sed):
+    p, s = sed
+    s = dict(s, bedload_law=S.BEDLOAD_GRASS, grass_ag=1e-3)
+    got = oracle.bedload_flux(p, s, 1.0, 0.6, 0.8)
+    assert got == pytest.approx(-1e-3 / (1 - p["porosity"]) * 0.6 * 1.0, rel=1e-14)
+
+
+def test_erosion_deposition_garcia_parker(oracle, sed):
+    p, s = sed
+    R, d, poro, eps_h = p["R"], p["DS"], p["porosity"], p["epsilon_h"]
+    nu = 1.002e-6
+    vs = zhang_settling(d, R, nu)
+    Rp = d * math.sqrt((R - 1) * G * d) / nu
+    a1, a2 = (1.0, 0.6) if Rp > 2.36 else (0.586, 1.23)
+    r0 = 1.64 + 0.4 * (d / p["Dsg"]) ** 1.64           # near-bed / depth-mean concentration ratio
+    for h, qx, qy, hc in [(1.0, 1.5, 0.0, 0.0), (0.3, 0.6, 0.6, 0.003), (2.0, 0.2, 0.0, 0.04), (1.0, 9.0, 0.0, 0.0)]:
+        u = math.hypot(qx, qy) / h
+        ustar = math.sqrt(p["FR"]) * u
+        Zu = a1 * Rp ** a2 * ustar / vs
+        A = 1.3e-7
+        # Garcia-Parker E = A Z^5 / (1 + (A/0.3) Z^5); the reference rounds
+        # A/0.3 = 4.33e-7 to 4.3e-7 (SedFlux.cuh:68)
+        Es = A * Zu ** 5 / (1 + 4.3e-7 * Zu ** 5)
+        Fe = min(vs * (1 - poro) * Es, 0.1)
+        Fd = min(vs * r0 * hc / (h + eps_h), 0.1)
+        got = oracle.erosion_rate(p, s, h, qx, qy, hc)
+        assert got == pytest.approx(Fe - Fd, rel=1e-9, abs=1e-18)
+
+
+# ------------------------------------------------------------- edge flux ----
+
+def flux_phys(h, qn, qt, hc):
+    u = qn / h if h > 0 else 0.0
+    return np.array([qn, qn * u, qt * u, 0.0, hc * u])
+
+
+def test_edge_consistency_equal_states(oracle, sed):
+    p, s = sed
+    s = dict(s, bedload=0)
+    st = np.array([1.7, 0.9, -0.4, 3.0, 0.02])
+    dm, dp = oracle.edge_flux(p, s, st, st, (1, 0), 1)
+    F = flux_phys(st[0] + 0.0, st[1], st[2], st[4])
+    u = st[1] / (st[0] + np.finfo(float).eps)
+    F = np.array([st[0] * u, st[0] * u * u, st[0] * u * (st[2] / (st[0] + np.finfo(float).eps)), 0, st[4] / (st[0] + np.finfo(float).eps) * st[0] * u])
+    np.testing.assert_allclose(dm, F, rtol=1e-14, atol=1e-300)
+    np.testing.assert_array_equal(dp, -dm)
+
+
+def test_edge_lake_at_rest_is_zero(oracle, sed):
+    p, s = sed
+    # dyadic values keep eta = h - H exact on both sides; last: emerged right cell
+    for Hl, Hr, eta in [(3.0, 1.0, 0.0), (2.0, 5.0, 0.5), (2.0, -1.0, 0.0), (7.25, 0.5, -0.25)]:
+        L = np.array([max(eta + Hl, 0), 0, 0, Hl, 0])
+        R = np.array([max(eta + Hr, 0), 0, 0, Hr, 0])
+        dm, dp = oracle.edge_flux(p, s, L, R, (1, 0), 1)
+        assert np.all(dm == 0) and np.all(dp == 0)
+
+
+def hll_textbook(hL, uL, vL, hR, uR, vR):
+    """HLL flux of the 2-D SWE (x-split) with Einfeldt-type speeds from the
+    Roe average; flux includes the pressure g h^2 / 2."""
+    cL, cR = math.sqrt(G * hL), math.sqrt(G * hR)
+    ut = (math.sqrt(hL) * uL + math.sqrt(hR) * uR) / (math.sqrt(hL) + math.sqrt(hR))
+    ct = math.sqrt(G * 0.5 * (hL + hR))
+    SL, SR = min(uL - cL, ut - ct), max(uR + cR, ut + ct)
+    UL = np.array([hL, hL * uL, hL * vL])
+    UR = np.array([hR, hR * uR, hR * vR])
+    FL = np.array([hL * uL, hL * uL * uL + 0.5 * G * hL * hL, hL * uL * vL])
+    FR = np.array([hR * uR, hR * uR * uR + 0.5 * G * hR * hR, hR * uR * vR])
+    if SL >= 0:
+        return FL
+    if SR <= 0:
+        return FR
+    return (SR * FL - SL * FR + SL * SR * (UR - UL)) / (SR - SL)
+
+
+@pytest.mark.parametrize("case", [(2.0, 0.5, 0.1, 1.0, -0.3, 0.2), (1.0, 0.0, 0.0, 3.0, 0.0, 0.0),
+                                  (0.5, 2.5, 0.0, 0.4, 2.2, 1.0), (3.0, -3.0, 0.3, 2.9, -2.5, 0.0)])
+def test_edge_flat_bed_matches_textbook_hll(oracle, sed, case):
+    p, s = sed
+    s = dict(s, bedload=0)
+    hL, uL, vL, hR, uR, vR = case
+    L = np.array([hL, hL * uL, hL * vL, 0.0, 0.0])
+    R = np.array([hR, hR * uR, hR * vR, 0.0, 0.0])
+    dm, dp = oracle.edge_flux(p, s, L, R, (1, 0), 0)
+    F = hll_textbook(hL, uL, vL, hR, uR, vR)
+    # the path-conservative form carries the pressure of the left state in the cell
+    assert dm[0] == pytest.approx(F[0], rel=1e-12, abs=1e-12)
+    assert dm[1] + 0.5 * G * hL * hL == pytest.approx(F[1], rel=1e-12)
+    assert dm[2] == pytest.approx(F[2], rel=1e-12, abs=1e-12)
+    # conservation identities of the fluctuations
+    assert dm[0] + dp[0] == pytest.approx(0.0, abs=1e-13 * max(1, abs(dm[0])))
+    hm = 0.5 * (hL + hR)
+    assert dm[1] + dp[1] == pytest.approx(G * hm * (hR - hL), rel=1e-12, abs=1e-12)
+
+
+def test_edge_mirror_and_rotation_symmetry(oracle, sed):
+    p, s = sed
+    rng = np.random.default_rng(3)
+    for _ in range(300):
+        L = np.array([rng.uniform(0, 3), rng.normal(0, 2), rng.normal(0, 2), rng.uniform(-1, 3), 0.0])
+        R = np.array([rng.uniform(0, 3), rng.normal(0, 2), rng.normal(0, 2), rng.uniform(-1, 3), 0.0])
+        L[4], R[4] = 0.01 * L[0], 0.02 * R[0]
+        fx = int(rng.integers(0, 2))
+        dm, dp = oracle.edge_flux(p, s, L, R, (1, 0), fx)
+        # mirror x -> -x: swap sides and negate qx; the fluctuation sent to
+        # the left cell becomes the one sent to the right cell (normal
+        # momentum odd, every other component even)
+        M = np.array([1, -1, 1, 1, 1.0])
+        dm2, dp2 = oracle.edge_flux(p, s, R * M, L * M, (1, 0), fx)
+        np.testing.assert_array_equal(dm2, dp * M)
+        np.testing.assert_array_equal(dp2, dm * M)
+        # rotation by 90 degrees: the y-edge of (qx, qy) is the x-edge of (qy, -qx)
+        dmy, dpy = oracle.edge_flux(p, s, L, R, (0, 1), fx)
+        Lr = np.array([L[0], L[2], -L[1], L[3], L[4]])
+        Rr = np.array([R[0], R[2], -R[1], R[3], R[4]])
+        dmx, dpx = oracle.edge_flux(p, s, Lr, Rr, (1, 0), fx)
+        np.testing.assert_array_equal(dmy, [dmx[0], -dmx[2], dmx[1], dmx[3], dmx[4]])
+        np.testing.assert_array_equal(dpy, [dpx[0], -dpx[2], dpx[1], dpx[3], dpx[4]])
+
+
+# ------------------------------------------------------------ dt reduction --
+
+def test_cell_dt_closed_form(oracle, sed):
+    p, _ = sed
+    h, qx, qy = 2.0, 3.0, -1.0
+    c = math.sqrt(G * h)
+    want = p["CFL"] * 0.5 * min(p["dx"] / (1.5 + c), p["dy"] / (0.5 + c))
+    assert oracle.cell_dt(p, h, qx, qy) == pytest.approx(want, rel=1e-15)
+    assert oracle.cell_dt(p, 0.0, 0.0, 0.0) == math.inf
+
+
+def test_min_reduction_narrowing():
+    import pyoracle
+    x = 0.1 + 1e-12                        # not representable in float
+    v = np.full(5000, 10.0)
+    v[1234] = x
+    v[10] = np.nan
+    assert pyoracle.min_reduction(v, 1) == float(np.float32(x))
+    assert pyoracle.min_reduction(v, 1) != x
+    assert pyoracle.min_reduction(v[:4096], 1) == x                # small grids: exact double minimum
+    assert pyoracle.min_reduction(v, 0) == x
+    assert pyoracle.min_reduction(np.full(5000, np.inf), 1) == 1e30   # all dry
+
+
+# ------------------------------------------------------------ grid level ----
+
+def test_uniform_periodic_state_friction_and_exchange_closed_form(oracle):
+    """A uniform state on a periodic grid has zero net flux, so one RK2 step is
+    the closed form q1 = q/(1+c), q2 = q1/(1+c), q' = (q + q2)/2 with
+    c = dt FR |u|/(h+eps) (friction uses U^n in both stages), and the bed/
+    suspension exchange moves dt*phi per stage from H to hc."""
+    n = 8
+    p = S.base_params(n, n, 10.0, 10.0, friction=0.025)
+    s = S.default_scheme(bc=[S.BC_PERIODIC] * 4, bedload=0)
+    h, q, H, hc = 1.3, 0.9, 4.0, 0.0
+    V = S.pack_state(np.full((n, n), h), np.full((n, n), q), np.zeros((n, n)), np.full((n, n), H),
+                     np.full((n, n), hc))
+    o = oracle.OracleSim(p, s, V, np.ones((n, n), np.int32))
+    dt = o.dt
+    eps = np.finfo(float).eps
+    u = q / (h + eps)
+    c = dt * p["FR"] * u / (h + eps)
+    phi = oracle.erosion_rate(p, s, h, q, 0.0, hc)
+    o.step(1)
+    W = o.state()
+    q2 = q / (1 + c) / (1 + c)
+    qn = 0.5 * q + 0.5 * q2
+    assert np.allclose(W[1], qn * h / (h + eps) * 1.0, rtol=1e-14)
+    assert np.all(W[0] == h)
+    # H: stage 1 H - dt phi; stage 2 (H - dt phi) - dt phi; RK average
+    Hn = 0.5 * H + 0.5 * (H - 2 * dt * phi)
+    assert np.allclose(W[3], Hn, rtol=1e-14, atol=1e-18)
+    assert np.allclose(W[3] + W[4], H + hc, rtol=1e-15)
+
+
+def test_lake_at_rest_grid(oracle):
+    sc = S.lake_at_rest(n=48)
+    o = oracle.from_scenario(sc)
+    o.step(20)
+    np.testing.assert_array_equal(o.state(), sc.V)
+
+
+def test_mass_conservation_walls(oracle):
+    # bedload only, over wet/dry terrain: water and bed volume are conserved
+    sc = S.gaussian_source(n=40, dx=50.0, seed=2, suspension=0)
+    o = oracle.from_scenario(sc)
+    m0, b0 = sc.V[0].sum(), sc.V[3].sum()
+    o.step(40)
+    W = o.state()
+    assert abs(W[0].sum() - m0) <= 1e-13 * m0
+    assert abs(W[3].sum() - b0) <= 1e-13 * abs(b0)
+    assert np.abs(W[3] - sc.V[3]).max() > 0   # the bed did move
+
+
+def test_sediment_conservation_deep_water(oracle):
+    # Suspension on: Sum(H + hc) is conserved while no cell is shallow enough
+    # for deposition to overdraw hc (the reference clips the negative hc in
+    # state_filter, SedFlux.cuh:123-125, which creates sediment there).
+    n = 40
+    p = S.base_params(n, n, 20.0, 20.0)
+    s = S.default_scheme()
+    x, y = S.cell_centres(n, n, 20.0, 20.0)
+    X, Y = np.meshgrid(x, y)
+    H = 6.0 + 0.5 * np.sin(X / 90.0) * np.cos(Y / 70.0)
+    h = H + 0.8 * np.exp(-((X - 300) ** 2 + (Y - 500) ** 2) / 2e4)
+    V = S.pack_state(h, 1.2 * h, 0.3 * h, H, 1e-3 * h)
+    o = oracle.OracleSim(p, s, V, S.nonerodible_mask(n, n, 7))
+    m0, s0 = V[0].sum(), (V[3] + V[4]).sum()
+    o.step(40)
+    W = o.state()
+    assert abs(W[0].sum() - m0) <= 1e-13 * m0
+    assert abs((W[3] + W[4]).sum() - s0) <= 1e-13 * s0
+    assert np.abs(W[4] - V[4]).max() > 1e-6   # sediment was exchanged
+
+
+def test_mirror_symmetry_plain_swe(oracle):
+    sc = S.plain_swe(n=48, dx=200.0)
+    o = oracle.from_scenario(sc)
+    o.step(30)
+    W = o.state()
+    np.testing.assert_array_equal(W[0], W[0][::-1])
+    np.testing.assert_array_equal(W[1], W[1][::-1])
+    np.testing.assert_array_equal(W[2], -W[2][::-1])
+    assert np.abs(W[1]).max() > 0
+
+
+def test_transpose_symmetry(oracle):
+    sc = S.gaussian_source(n=32, dx=60.0, seed=4)
+    V = sc.V
+    Vt = np.ascontiguousarray(np.stack([V[0].T, V[2].T, V[1].T, V[3].T, V[4].T]))
+    a = oracle.OracleSim(sc.params, sc.scheme, V, sc.fixH)
+    b = oracle.OracleSim(sc.params, sc.scheme, Vt, np.ascontiguousarray(sc.fixH.T))
+    a.step(25)
+    b.step(25)
+    A, B = a.state(), b.state()
+    np.testing.assert_array_equal(A, np.stack([B[0].T, B[2].T, B[1].T, B[3].T, B[4].T]))
+
+
+def ritter(x, t, h0):
+    c0 = math.sqrt(G * h0)
+    h = np.where(x <= -c0 * t, h0, np.where(x >= 2 * c0 * t, 0.0, (2 * c0 - x / t) ** 2 / (9 * G)))
+    return h
+
+
+def dambreak_error(n, h0=1.0, L=100.0, T=4.0):
+    p = S.base_params(n, 1, L / n, L / n, friction=0.0)
+    p["CFL"] = 0.45
+    s = S.default_scheme(bedload=0, suspension=0, bc=[S.BC_WALL] * 4)
+    x = (np.arange(n) + 0.5) * L / n - L / 2
+    h = np.where(x < 0, h0, 0.0)[None, :]
+    V = S.pack_state(h, np.zeros_like(h), np.zeros_like(h), np.zeros_like(h))
+    import pyoracle
+    o = pyoracle.OracleSim(p, s, V, np.ones((1, n), np.int32))
+    while o.t < T:
+        o.step(1)
+    W = o.state()[0, 0]
+    return np.abs(W - ritter(x, o.t, h0)).mean(), o.t
+
+
+def test_ritter_dam_break_convergence():
+    e1, _ = dambreak_error(200)
+    e2, _ = dambreak_error(800)
+    assert e1 < 0.02
+    assert e2 < 0.6 * e1          # converges to the analytic dry-bed solution
