@@ -78,7 +78,7 @@ def run_variant(sc, precision, steps, warmup, tile_rows):
         "stage_ms_mean": [ms[0] / max(launches[0], 1), ms[1] / max(launches[1], 1)],
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS, "traffic": None,
-                     "kernel": "stage_kernel<%s, true>" % ("double" if precision == "fp64" else "float"),
+                     "kernel": "stage_kernel<%s, true>" % ("double, double" if precision == "fp64" else "float, float"),
                      "algorithmic_bytes_per_launch": per_launch_bytes},
         "mass": g.mass(),
     }
@@ -179,7 +179,8 @@ def main():
         "roofline": r64["roofline"],
         "cpu_baseline": cpu,
         "fp32": None if r32 is None else {"value": r32["value"], "ms_per_step": r32["ms_per_step"],
-                                          "dtype": "f32 planes (f64 arithmetic)", "bytes_per_cell_update": 100,
+                                          "dtype": "f32 state and fluxes, compensated h/H/hc (+60 B/cell-update lo planes)",
+                                          "bytes_per_cell_update": 100,
                                           "roofline": r32["roofline"]},
         "stage_ms_mean": r64["stage_ms_mean"],
         "active_flow": active,

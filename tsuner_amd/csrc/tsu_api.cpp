@@ -180,6 +180,10 @@ struct DeviceGuard {
     }
 };
 
+// FP64: h, qx, qy, H, hc.  FP32: the same five planes plus the low-order
+// parts of h, H and hc (compensated accumulation).
+size_t planes_of(const tsu_state* st) { return st->precision == TSU_FP64 ? 5 : 8; }
+
 int default_tile_rows(const tsu_state* st)
 {
     // Enough waves to cover the 1024 SIMDs a few times, tall enough to keep
@@ -359,7 +363,7 @@ TSU_API int tsu_state_create(tsu_state** out, const tsu_sim_params* p, const tsu
     set_tiling(st, default_tile_rows(st));
 
     DeviceGuard g(device);
-    const size_t state_bytes = 5 * (size_t)st->nvol * st->esize;
+    const size_t state_bytes = planes_of(st) * (size_t)st->nvol * st->esize;
     st->mass_nb = tsu::mass_blocks(st->nvol);
     hipError_t e = hipStreamCreateWithFlags(&st->stream, hipStreamNonBlocking);
     if (e == hipSuccess) e = hipMalloc(&st->A, state_bytes);
@@ -409,12 +413,19 @@ TSU_API int tsu_state_upload(tsu_state* st, const double* V, const int* fixH)
     st->sed = st->s.bedload || st->s.suspension || !hc_zero;
     DeviceGuard g(st->device);
     drop_graph(st);
-    const size_t bytes = 5 * (size_t)nv * st->esize;
+    const size_t bytes = planes_of(st) * (size_t)nv * st->esize;
     if (st->precision == TSU_FP64) {
         HIPCHK(hipMemcpyAsync(st->A, V, bytes, hipMemcpyHostToDevice, st->stream));
     } else {
-        std::vector<float> f((size_t)5 * nv);
-        for (size_t i = 0; i < f.size(); i++) f[i] = (float)V[i];
+        // hi planes 0-4, then the lo parts of h, H, hc (planes 5-7)
+        std::vector<float> f((size_t)8 * nv);
+        for (size_t i = 0; i < (size_t)5 * nv; i++) f[i] = (float)V[i];
+        const int src[3] = {0, 3, 4};
+        for (int j = 0; j < 3; j++)
+            for (size_t i = 0; i < (size_t)nv; i++) {
+                const size_t k = (size_t)src[j] * nv + i;
+                f[(size_t)(5 + j) * nv + i] = (float)(V[k] - (double)f[k]);
+            }
         HIPCHK(hipMemcpyAsync(st->A, f.data(), bytes, hipMemcpyHostToDevice, st->stream));
         HIPCHK(hipStreamSynchronize(st->stream));
     }
@@ -440,15 +451,18 @@ TSU_API int tsu_state_download(tsu_state* st, double* V)
     int rc = current_buffer(st, &cur);
     if (rc) return rc;
     const long long nv = st->nvol;
-    const size_t bytes = 5 * (size_t)nv * st->esize;
+    const size_t bytes = planes_of(st) * (size_t)nv * st->esize;
     if (st->precision == TSU_FP64) {
         HIPCHK(hipMemcpyAsync(V, cur, bytes, hipMemcpyDeviceToHost, st->stream));
         HIPCHK(hipStreamSynchronize(st->stream));
     } else {
-        std::vector<float> f((size_t)5 * nv);
+        std::vector<float> f((size_t)8 * nv);
         HIPCHK(hipMemcpyAsync(f.data(), cur, bytes, hipMemcpyDeviceToHost, st->stream));
         HIPCHK(hipStreamSynchronize(st->stream));
-        for (size_t i = 0; i < f.size(); i++) V[i] = (double)f[i];
+        for (size_t i = 0; i < (size_t)5 * nv; i++) V[i] = (double)f[i];
+        const int dst[3] = {0, 3, 4};
+        for (int j = 0; j < 3; j++)
+            for (size_t i = 0; i < (size_t)nv; i++) V[(size_t)dst[j] * nv + i] += (double)f[(size_t)(5 + j) * nv + i];
     }
     return TSU_OK;
 }

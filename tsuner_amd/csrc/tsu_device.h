@@ -2,18 +2,29 @@
 // gfx950 (CDNA4), shared by the fused stage kernels and the known-answer
 // kernels.
 //
-// Every function computes the same IEEE value as the reference expression it
-// cites (no FMA contraction: the library is built with -ffp-contract=off, as
-// the reference is built with -fmad=false, GPU/makefile:10).  Where this code
-// is shaped differently from the reference, the change is value-preserving:
+// Every function is a template on the compute type R.
+//
+// R = double is the reference arithmetic: each function computes the same
+// IEEE value as the reference expression it cites (no FMA contraction: the
+// library is built with -ffp-contract=off, as the reference is built with
+// -fmad=false, GPU/makefile:10).  Where this code is shaped differently from
+// the reference, the change is value-preserving:
 //   * products with a 0/1 gate, e.g. x*(h>=EPS), become a select: the reference
 //     yields +-0 for a closed gate, this code yields +0 (equal as values);
 //   * rotations by the axis-aligned edge normals are signed permutations
 //     (q*1 + q'*0 equals q as a value);
 //   * a quantity the reference recomputes from identical operands (the cell
-//     velocity, |u|) is computed once.
+//     velocity, |u|) is computed once;
+//   * divisions whose numerator is exactly zero are skipped (0/d == 0).
 // Results therefore equal the reference's as values (+0 == -0), which is how
 // the parity tests compare them.
+//
+// R = float is the FP32 solver: the same scheme with single-instruction
+// reciprocal and square root (v_rcp_f32, v_sqrt_f32, about 1 ulp), the
+// velocity of near-dry cells (h < 1 cm) evaluated in FP64 exactly as the
+// reference does, and mass, bed and suspension fluctuations made exactly
+// antisymmetric across each edge (Dp = -Dm) so that compensated cell
+// accumulation conserves them to FP64-level round-off.
 #pragma once
 
 #include <hip/hip_runtime.h>
@@ -26,6 +37,8 @@ constexpr double kEpsilon = 2.2204460492503131e-16;  // DBL_EPSILON
 constexpr double kG = 9.81;
 constexpr double kEps = 1e-6;
 constexpr double kVmax = 10.0;
+// FP32 solver: cells shallower than this get their velocity in FP64.
+constexpr float kNearDry = 1e-2f;
 
 enum { BC_FREE = 0, BC_WALL = 1, BC_PERIODIC = 2, BC_INFLOW = 3 };
 enum { LIM_VANLEER = 0, LIM_MINMOD = 1 };
@@ -70,30 +83,50 @@ struct Ctrl {
     long long pad[3];
 };
 
-__device__ __forceinline__ int sgn(double x) { return (fabs(x) < kEpsilon) ? 0 : ((x > 0) ? 1 : -1); }
+// ---- primitive operations per compute type --------------------------------
+
+__device__ __forceinline__ double rdiv(double a, double b) { return a / b; }
+__device__ __forceinline__ float rdiv(float a, float b) { return a * __builtin_amdgcn_rcpf(b); }
+__device__ __forceinline__ double rsqrt_(double x) { return sqrt(x); }
+__device__ __forceinline__ float rsqrt_(float x) { return __builtin_amdgcn_sqrtf(x); }
+__device__ __forceinline__ double rmin(double a, double b) { return fmin(a, b); }
+__device__ __forceinline__ float rmin(float a, float b) { return fminf(a, b); }
+__device__ __forceinline__ double rmax(double a, double b) { return fmax(a, b); }
+__device__ __forceinline__ float rmax(float a, float b) { return fmaxf(a, b); }
+__device__ __forceinline__ double rabs(double a) { return fabs(a); }
+__device__ __forceinline__ float rabs(float a) { return fabsf(a); }
+
+template <typename R>
+__device__ __forceinline__ int sgn(R x)
+{
+    return (rabs(x) < (R)kEpsilon) ? 0 : ((x > (R)0) ? 1 : -1);
+}
 
 // Reconstructor.cu:19-31.  When a and b do not share a strict sign the
 // numerator a|b| + b|a| is exactly zero (for finite products), so the
 // division is only evaluated for same-sign slopes.
-__device__ __forceinline__ double lim_vanleer(double a, double b)
+template <typename R>
+__device__ __forceinline__ R lim_vanleer(R a, R b)
 {
-    double aa = fabs(a), bb = fabs(b);
-    double s = aa + bb;
-    double d = 0.0;
-    if ((a > 0.0 && b > 0.0) || (a < 0.0 && b < 0.0)) {
-        if (s > kEpsilon) d = (a * bb + b * aa) / s;
+    R aa = rabs(a), bb = rabs(b);
+    R s = aa + bb;
+    R d = (R)0;
+    if ((a > (R)0 && b > (R)0) || (a < (R)0 && b < (R)0)) {
+        if (s > (R)kEpsilon) d = rdiv(a * bb + b * aa, s);
     }
     return d;
 }
 
-__device__ __forceinline__ double lim_minmod(double a, double b)
+template <typename R>
+__device__ __forceinline__ R lim_minmod(R a, R b)
 {
-    if (a > 0.0 && b > 0.0) return fmin(a, b);
-    if (a < 0.0 && b < 0.0) return fmax(a, b);
-    return 0.0;
+    if (a > (R)0 && b > (R)0) return rmin(a, b);
+    if (a < (R)0 && b < (R)0) return rmax(a, b);
+    return (R)0;
 }
 
-__device__ __forceinline__ double limiter(int law, double a, double b)
+template <typename R>
+__device__ __forceinline__ R limiter(int law, R a, R b)
 {
     return law == LIM_MINMOD ? lim_minmod(a, b) : lim_vanleer(a, b);
 }
@@ -102,20 +135,22 @@ __device__ __forceinline__ double limiter(int law, double a, double b)
 // s > nextafter(100, +inf).
 constexpr double kVmax2Cut = 100.00000000000001421085471520200371742248535156250;
 
+template <typename R>
 struct Vel {
-    double x, y;
-    double s;   // u.x*u.x + u.y*u.y of the returned (possibly capped) velocity
+    R x, y;
+    R s;   // u.x*u.x + u.y*u.y of the returned (possibly capped) velocity
 };
 
 // |u| as the reference evaluates it after u_calculator (Reconstructor.cu:43,74;
 // SedFlux.cuh:157): sqrt(u.x*u.x + u.y*u.y).
-__device__ __forceinline__ double speed(const Vel& u) { return sqrt(u.s); }
+template <typename R>
+__device__ __forceinline__ R speed(const Vel<R>& u) { return rsqrt_(u.s); }
 
-// SedFlux.cuh:26-49.  Divisions with a zero numerator are skipped (0/d is
-// 0); the cap test uses s instead of sqrt(s) (see kVmax2Cut).
-__device__ __forceinline__ Vel velocity(double h, double qx, double qy)
+// SedFlux.cuh:26-49 in FP64.  Divisions with a zero numerator are skipped
+// (0/d is 0); the cap test uses s instead of sqrt(s) (see kVmax2Cut).
+__device__ __forceinline__ Vel<double> velocity64(double h, double qx, double qy)
 {
-    Vel u;
+    Vel<double> u;
     u.x = 0.0;
     u.y = 0.0;
     if (h >= kEps) {
@@ -134,17 +169,45 @@ __device__ __forceinline__ Vel velocity(double h, double qx, double qy)
     return u;
 }
 
-// SedFlux.cuh:118-132
-template <bool SED>
-__device__ __forceinline__ void state_filter(double& h, double& qx, double& qy, double& hc)
+__device__ __forceinline__ Vel<double> velocity(double h, double qx, double qy) { return velocity64(h, qx, qy); }
+
+// FP32 solver: near-dry cells take the reference's FP64 desingularisation,
+// wet cells a single reciprocal.
+__device__ __forceinline__ Vel<float> velocity(float h, float qx, float qy)
 {
-    h = (h > 0.0) ? h : 0.0;
-    double c = 0.0;
-    if (SED) {
-        if (hc != 0.0) c = hc / (h + kEpsilon);
-        c = (c > 0.0) ? c : 0.0;
+    Vel<float> u;
+    if (h < kNearDry) {
+        Vel<double> d = velocity64((double)h, (double)qx, (double)qy);
+        u.x = (float)d.x;
+        u.y = (float)d.y;
+        u.s = (float)d.s;
+        return u;
     }
-    Vel u = velocity(h, qx, qy);
+    float r = __builtin_amdgcn_rcpf(h);
+    u.x = qx * r;
+    u.y = qy * r;
+    float s = u.x * u.x + u.y * u.y;
+    if (s > 100.0f) {
+        float k = (float)kVmax * __builtin_amdgcn_rsqf(s);
+        u.x *= k;
+        u.y *= k;
+        s = u.x * u.x + u.y * u.y;
+    }
+    u.s = s;
+    return u;
+}
+
+// SedFlux.cuh:118-132
+template <typename R, bool SED>
+__device__ __forceinline__ void state_filter(R& h, R& qx, R& qy, R& hc)
+{
+    h = (h > (R)0) ? h : (R)0;
+    R c = (R)0;
+    if (SED) {
+        if (hc != (R)0) c = rdiv(hc, h + (R)kEpsilon);
+        c = (c > (R)0) ? c : (R)0;
+    }
+    Vel<R> u = velocity(h, qx, qy);
     qx = h * u.x;
     qy = h * u.y;
     if (SED) hc = h * c;
@@ -152,21 +215,23 @@ __device__ __forceinline__ void state_filter(double& h, double& qx, double& qy, 
 
 // Bed-plane flux before the mask: SedFlux.cuh:150-168 (Ashida-Michiue), or
 // Grass q_b = A_g/(1-p) u_n |u|^2.  Sign as in the reference (returns -q_b).
-__device__ __forceinline__ double bedload(const DevParams& P, double h, const Vel& u)
+template <typename R>
+__device__ __forceinline__ R bedload(const DevParams& P, R h, const Vel<R>& u)
 {
-    if (!(h > P.HQB && h > P.fiveDS)) return 0.0;
-    double m = speed(u);
-    if (P.bedload_law == BEDLOAD_GRASS) return -(P.grass_k * u.x * m * m);
-    double O = P.Opar * m * m;
-    if (!(O > P.OC)) return 0.0;
-    double K1 = 17.0 * u.x / (m + kEpsilon);
-    double qb = K1 * P.Q * (O - P.OC) * fmax(sqrt(O) - P.sqrtOC, 0.0);
+    if (!(h > (R)P.HQB && h > (R)P.fiveDS)) return (R)0;
+    R m = speed(u);
+    if (P.bedload_law == BEDLOAD_GRASS) return -((R)P.grass_k * u.x * m * m);
+    R O = (R)P.Opar * m * m;
+    if (!(O > (R)P.OC)) return (R)0;
+    R K1 = rdiv((R)17.0 * u.x, m + (R)kEpsilon);
+    R qb = K1 * (R)P.Q * (O - (R)P.OC) * rmax(rsqrt_(O) - (R)P.sqrtOC, (R)0);
     return -qb;
 }
 
 // Unrotated (qx, qy) -> normal frame (qn, qt) for the four axis normals
 // (VolumeSolver.cu:28-31 with normal_x, normal_y in {-1, 0, 1}).
-__device__ __forceinline__ void to_normal(int kind, double qx, double qy, double& qn, double& qt)
+template <typename R>
+__device__ __forceinline__ void to_normal(int kind, R qx, R qy, R& qn, R& qt)
 {
     switch (kind) {
     case N_PX: qn = qx; qt = qy; break;
@@ -177,7 +242,8 @@ __device__ __forceinline__ void to_normal(int kind, double qx, double qy, double
 }
 
 // Normal frame -> unrotated (VolumeSolver.cu:142-152).
-__device__ __forceinline__ void from_normal(int kind, double a, double b, double& fx, double& fy)
+template <typename R>
+__device__ __forceinline__ void from_normal(int kind, R a, R b, R& fx, R& fy)
 {
     switch (kind) {
     case N_PX: fx = a; fy = b; break;
@@ -192,120 +258,137 @@ __device__ __forceinline__ void from_normal(int kind, double a, double b, double
 // the right cell).  Inputs: left/right face (h, H, qn, qt, hc).  With SED
 // false the bed and concentration rows are identically zero (no bedload, no
 // suspension, hc == 0) and are not computed.
-template <bool SED>
+template <typename R, bool SED>
 __device__ __forceinline__ void riemann(const DevParams& P, bool bed_mobile,
-                                        double hl, double zl, double qnl, double qtl, double hcl,
-                                        double hr, double zr, double qnr, double qtr, double hcr,
-                                        double* Dm, double* Dp)
+                                        R hl, R zl, R qnl, R qtl, R hcl,
+                                        R hr, R zr, R qnr, R qtr, R hcr,
+                                        R* Dm, R* Dp)
 {
-    double zmH = fmin(zl, zr);
-    double h0 = fmax(hl - zl + zmH, 0.0);
-    double h1 = fmax(hr - zr + zmH, 0.0);
-    if (h0 == 0.0 && h1 == 0.0) {
+    const R g = (R)kG, half = (R)0.5, zero = (R)0;
+    R zmH = rmin(zl, zr);
+    R h0 = rmax(hl - zl + zmH, zero);
+    R h1 = rmax(hr - zr + zmH, zero);
+    if (h0 == zero && h1 == zero) {
         // Both reconstructed depths vanish: every flux, jump and bed term
         // below is a product with h0, h1, their difference or a zero jump,
         // so Dm = Dp = 0 (the speeds stay finite: |u| <= VMAX).
-        for (int f = 0; f < (SED ? 5 : 3); f++) Dm[f] = Dp[f] = 0.0;
+        for (int f = 0; f < (SED ? 5 : 3); f++) Dm[f] = Dp[f] = zero;
         return;
     }
-    Vel u0 = velocity(hl, qnl, qtl);
-    Vel u1 = velocity(hr, qnr, qtr);
-    double c0 = 0.0, c1 = 0.0;
+    Vel<R> u0 = velocity(hl, qnl, qtl);
+    Vel<R> u1 = velocity(hr, qnr, qtr);
+    R c0 = zero, c1 = zero;
     if (SED) {
-        if (hcl != 0.0) c0 = hcl / (hl + kEpsilon);
-        if (hcr != 0.0) c1 = hcr / (hr + kEpsilon);
+        if (hcl != zero) c0 = rdiv(hcl, hl + (R)kEpsilon);
+        if (hcr != zero) c1 = rdiv(hcr, hr + (R)kEpsilon);
     }
-    double q0n = u0.x * h0, q0t = u0.y * h0;
-    double q1n = u1.x * h1, q1t = u1.y * h1;
-    double deta = h1 - h0;
+    R q0n = u0.x * h0, q0t = u0.y * h0;
+    R q1n = u1.x * h1, q1t = u1.y * h1;
+    R deta = h1 - h0;
 
-    double F00 = h0 * u0.x, F01 = F00 * u0.x, F02 = F00 * u0.y;
-    double F10 = h1 * u1.x, F11 = F10 * u1.x, F12 = F10 * u1.y;
+    R F00 = h0 * u0.x, F01 = F00 * u0.x, F02 = F00 * u0.y;
+    R F10 = h1 * u1.x, F11 = F10 * u1.x, F12 = F10 * u1.y;
 
     // Roe average (SedFlux.cuh:181-189) and HLL speeds (VolumeSolver.cu:63-73)
-    double hm = 0.5 * (h0 + h1);
-    double SL = u0.x - sqrt(kG * h0);
-    double SR = u1.x + sqrt(kG * h1);
-    double cm = sqrt(kG * hm);
-    double sh0 = sqrt(h0), sh1 = sqrt(h1);
-    double ssum = sh0 + sh1;
-    if (ssum > 0.0) {  // both sides dry: the reference's 0/0 is dropped by fmin/fmax
-        double umx = (sh0 * u0.x + sh1 * u1.x) / ssum;
-        SL = fmin(SL, umx - cm);
-        SR = fmax(SR, umx + cm);
+    R hm = half * (h0 + h1);
+    R SL = u0.x - rsqrt_(g * h0);
+    R SR = u1.x + rsqrt_(g * h1);
+    R cm = rsqrt_(g * hm);
+    R sh0 = rsqrt_(h0), sh1 = rsqrt_(h1);
+    R ssum = sh0 + sh1;
+    if (ssum > zero) {  // both sides dry: the reference's 0/0 is dropped by fmin/fmax
+        R umx = rdiv(sh0 * u0.x + sh1 * u1.x, ssum);
+        SL = rmin(SL, umx - cm);
+        SR = rmax(SR, umx + cm);
     }
 
-    double a0, a1;
-    double dS = SR - SL;
-    if (dS > kEpsilon) {
-        a0 = (SR * fabs(SL) - SL * fabs(SR)) / dS;
-        a1 = (fabs(SR) - fabs(SL)) / dS;
+    R a0, a1;
+    R dS = SR - SL;
+    if (dS > (R)kEpsilon) {
+        if constexpr (sizeof(R) == 8) {
+            a0 = (SR * fabs(SL) - SL * fabs(SR)) / dS;
+            a1 = (fabs(SR) - fabs(SL)) / dS;
+        } else {
+            R r = __builtin_amdgcn_rcpf(dS);
+            a0 = (SR * rabs(SL) - SL * rabs(SR)) * r;
+            a1 = (rabs(SR) - rabs(SL)) * r;
+        }
     } else {
-        a0 = fmax(fabs(SL), fabs(SR));
-        a1 = 0.0;
+        a0 = rmax(rabs(SL), rabs(SR));
+        a1 = zero;
     }
 
-    double df0 = F10 - F00, df1 = F11 - F01, df2 = F12 - F02;
-    df1 += kG * hm * deta;
-    double DU0 = h1 - h0, DU1 = q1n - q0n, DU2 = q1t - q0t;
-    double v0 = a0 * DU0 + a1 * df0;
-    double v1 = a0 * DU1 + a1 * df1;
-    double v2 = a0 * DU2 + a1 * df2;
-    Dm[0] = 0.5 * (df0 - v0) + F00;
-    Dm[1] = 0.5 * (df1 - v1) + F01;
-    Dm[2] = 0.5 * (df2 - v2) + F02;
-    Dp[0] = 0.5 * (df0 + v0) - F10;
-    Dp[1] = 0.5 * (df1 + v1) - F11;
-    Dp[2] = 0.5 * (df2 + v2) - F12;
+    R df0 = F10 - F00, df1 = F11 - F01, df2 = F12 - F02;
+    df1 += g * hm * deta;
+    R DU0 = h1 - h0, DU1 = q1n - q0n, DU2 = q1t - q0t;
+    R v0 = a0 * DU0 + a1 * df0;
+    R v1 = a0 * DU1 + a1 * df1;
+    R v2 = a0 * DU2 + a1 * df2;
+    Dm[0] = half * (df0 - v0) + F00;
+    Dm[1] = half * (df1 - v1) + F01;
+    Dm[2] = half * (df2 - v2) + F02;
+    if constexpr (sizeof(R) == 8)
+        Dp[0] = half * (df0 + v0) - F10;
+    else
+        Dp[0] = -Dm[0];
+    Dp[1] = half * (df1 + v1) - F11;
+    Dp[2] = half * (df2 + v2) - F12;
 
     if (SED) {
-        double hc0 = h0 * c0, hc1 = h1 * c1;
-        double F03 = 0.0, F13 = 0.0;
+        R hc0 = h0 * c0, hc1 = h1 * c1;
+        R F03 = zero, F13 = zero;
         if (bed_mobile) {
             F03 = bedload(P, h0, u0);
             F13 = bedload(P, h1, u1);
         }
-        double F04 = hc0 * u0.x, F14 = hc1 * u1.x;
-        double df3 = F13 - F03, df4 = F14 - F04;
-        double DU4 = hc1 - hc0;
+        R F04 = hc0 * u0.x, F14 = hc1 * u1.x;
+        R df3 = F13 - F03, df4 = F14 - F04;
+        R DU4 = hc1 - hc0;
         int s0 = sgn(zr - zl), s1 = sgn(df3), s2 = sgn(a1);
         s1 = (s0 != 0) ? s1 * s0 : 0;
-        double v3 = (s2 == s1) ? a1 * df3 : -a1 * df3;
-        double v4 = a0 * DU4 + a1 * df4;
-        Dm[3] = 0.5 * (df3 - v3) + F03;
-        Dm[4] = 0.5 * (df4 - v4) + F04;
-        Dp[3] = 0.5 * (df3 + v3) - F13;
-        Dp[4] = 0.5 * (df4 + v4) - F14;
+        R v3 = (s2 == s1) ? a1 * df3 : -a1 * df3;
+        R v4 = a0 * DU4 + a1 * df4;
+        Dm[3] = half * (df3 - v3) + F03;
+        Dm[4] = half * (df4 - v4) + F04;
+        if constexpr (sizeof(R) == 8) {
+            Dp[3] = half * (df3 + v3) - F13;
+            Dp[4] = half * (df4 + v4) - F14;
+        } else {
+            Dp[3] = -Dm[3];
+            Dp[4] = -Dm[4];
+        }
     }
 }
 
 // Reconstructor.cu:57-88 with SedFlux.cuh:51-99: exchange rate phi between
 // the bed (H) and the suspended load (hc), evaluated on U^n.
-__device__ __forceinline__ double erosion_rate(const DevParams& P, double h, double modu, double hcv)
+template <typename R>
+__device__ __forceinline__ R erosion_rate(const DevParams& P, R h, R modu, R hcv)
 {
-    double c = 0.0;
-    if (hcv != 0.0) c = hcv / (h + P.epsilon_h);
-    double Z = P.ka * (modu * P.sqrtFR) / P.vs;
-    double Z5 = Z * Z * Z * Z * Z;
-    double Es = (1.3e-7 * Z5) / (1 + 4.3e-7 * Z5);
-    double Fe = P.vs1p * Es;
-    if (Fe > 0.1) Fe = 0.1;
-    Fe = (Fe > 0.0) ? Fe : 0.0;
-    double Cb = P.cbk * c;
-    double Fd = P.vs * Cb;
-    if (Fd > 0.1) Fd = 0.1;
-    Fd = (Fd > 0.0) ? Fd : 0.0;
+    R c = (R)0;
+    if (hcv != (R)0) c = rdiv(hcv, h + (R)P.epsilon_h);
+    R Z = rdiv((R)P.ka * (modu * (R)P.sqrtFR), (R)P.vs);
+    R Z5 = Z * Z * Z * Z * Z;
+    R Es = rdiv((R)1.3e-7 * Z5, (R)1 + (R)4.3e-7 * Z5);
+    R Fe = (R)P.vs1p * Es;
+    if (Fe > (R)0.1) Fe = (R)0.1;
+    Fe = (Fe > (R)0) ? Fe : (R)0;
+    R Cb = (R)P.cbk * c;
+    R Fd = (R)P.vs * Cb;
+    if (Fd > (R)0.1) Fd = (R)0.1;
+    Fd = (Fd > (R)0) ? Fd : (R)0;
     return Fe - Fd;
 }
 
 // Volume_kernel.cu:19-43
-__device__ __forceinline__ double cell_dt(const DevParams& P, double h, double qx, double qy)
+template <typename R>
+__device__ __forceinline__ R cell_dt(const DevParams& P, R h, R qx, R qy)
 {
-    Vel u = velocity(h, qx, qy);
-    double cg = sqrt(kG * h);
-    double lx = fabs(u.x) + cg;
-    double ly = fabs(u.y) + cg;
-    return P.cfl_half * fmin(P.dx / lx, P.dy / ly);
+    Vel<R> u = velocity(h, qx, qy);
+    R cg = rsqrt_((R)kG * h);
+    R lx = rabs(u.x) + cg;
+    R ly = rabs(u.y) + cg;
+    return (R)P.cfl_half * rmin(rdiv((R)P.dx, lx), rdiv((R)P.dy, ly));
 }
 
 // Reduction_kernel_min.cu:155-196 applied to the exact minimum: for
@@ -360,17 +443,21 @@ __device__ __forceinline__ void inflow_state(const DevParams& P, double ct, long
 // analogues): copy of the inner face, normal momentum component ax (1 = qx on
 // x sides, 2 = qy on y sides) negated for walls, inflow series while
 // t < tCC[timeCC-2] and a wall afterwards.
-__device__ __forceinline__ void ghost_state(const DevParams& P, int side, const double* in, double* g,
-                                            long long idx, int ax, double ct)
+template <typename R>
+__device__ __forceinline__ void ghost_state(const DevParams& P, int side, const R* in, R* g, long long idx, int ax,
+                                            double ct)
 {
     int type = P.bc[side];
     for (int f = 0; f < 5; f++) g[f] = in[f];
     if (type == BC_WALL) g[ax] = -in[ax];
     if (type == BC_INFLOW) {
-        if (ct < P.tcc_end)
-            inflow_state(P, ct, idx, g);
-        else
+        if (ct < P.tcc_end) {
+            double gd[5];
+            inflow_state(P, ct, idx, gd);
+            for (int f = 0; f < 5; f++) g[f] = (R)gd[f];
+        } else {
             g[ax] = -in[ax];
+        }
     }
 }
 

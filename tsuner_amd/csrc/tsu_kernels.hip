@@ -16,6 +16,11 @@
 //     atomicMin per wave (Reduction_kernel_min.cu:155-196), so dt never
 //     leaves the device.
 //
+// Two arithmetic variants share the code: R = double (reference arithmetic,
+// FP64 planes) and R = float (FP32 planes and fluxes; h, H and hc carried as
+// hi + lo float pairs updated with TwoSum, the lo planes stored after the
+// five state planes).
+//
 // Workgroups are dealt round-robin over the 8 XCDs; tile ids are remapped so
 // every XCD receives a contiguous band of tiles (neighbouring tiles share
 // halo rows and partial cache lines through that XCD's L2).
@@ -47,6 +52,8 @@ __device__ __forceinline__ int xcd_tile(int b, int nb)
 
 __device__ __forceinline__ double up1(double v) { return __shfl_up(v, 1, 64); }
 __device__ __forceinline__ double dn1(double v) { return __shfl_down(v, 1, 64); }
+__device__ __forceinline__ float up1(float v) { return __shfl_up(v, 1, 64); }
+__device__ __forceinline__ float dn1(float v) { return __shfl_down(v, 1, 64); }
 __device__ __forceinline__ int up1i(int v) { return __shfl_up(v, 1, 64); }
 
 __device__ __forceinline__ double wave_min(double v)
@@ -55,84 +62,127 @@ __device__ __forceinline__ double wave_min(double v)
     return v;
 }
 
-struct Cell {
-    double h, qx, qy, H, hc;   // state
-    double ux, uy, us, c;      // velocity, |u|^2 and concentration
-};
-
-template <typename S, bool SED>
-__device__ __forceinline__ void load_cell(const S* in, const S* Hin, long long nv, long long off, Cell& v)
+// Knuth's TwoSum: s + e == a + b exactly.
+__device__ __forceinline__ void two_sum(float a, float b, float& s, float& e)
 {
-    v.h = (double)in[off];
-    v.qx = (double)in[nv + off];
-    v.qy = (double)in[2 * nv + off];
-    v.H = (double)Hin[off];
-    v.hc = SED ? (double)in[4 * nv + off] : 0.0;
+    s = a + b;
+    float bb = s - a;
+    e = (a - (s - bb)) + (b - bb);
 }
 
-template <bool SED>
-__device__ __forceinline__ void derive_cell(Cell& v)
+// Compensated value hi + lo.
+struct Pair {
+    float hi, lo;
+};
+
+// Adds x into p exactly (the rounding error joins lo).
+__device__ __forceinline__ void pacc(Pair& p, float x)
 {
-    Vel u = velocity(v.h, v.qx, v.qy);
+    float s, e;
+    two_sum(p.hi, x, s, e);
+    p.hi = s;
+    p.lo += e;
+}
+
+__device__ __forceinline__ Pair pnorm(Pair p)
+{
+    Pair r;
+    two_sum(p.hi, p.lo, r.hi, r.lo);
+    return r;
+}
+
+// (a + b) / 2 in pair arithmetic (Volume_kernel.cu:244-253 with coef 1/2).
+__device__ __forceinline__ Pair pavg(Pair a, Pair b)
+{
+    Pair r;
+    two_sum(a.hi, b.hi, r.hi, r.lo);
+    r.lo += a.lo + b.lo;
+    r = pnorm(r);
+    r.hi *= 0.5f;
+    r.lo *= 0.5f;
+    return r;
+}
+
+template <typename R>
+struct Cell {
+    R h, qx, qy, H, hc;   // state (hi part in the FP32 solver)
+    R ux, uy, us, c;      // velocity, |u|^2 and concentration
+};
+
+template <typename R, typename S, bool SED>
+__device__ __forceinline__ void load_cell(const S* in, const S* Hin, long long nv, long long off, Cell<R>& v)
+{
+    v.h = (R)in[off];
+    v.qx = (R)in[nv + off];
+    v.qy = (R)in[2 * nv + off];
+    v.H = (R)Hin[off];
+    v.hc = SED ? (R)in[4 * nv + off] : (R)0;
+}
+
+template <typename R, bool SED>
+__device__ __forceinline__ void derive_cell(Cell<R>& v)
+{
+    Vel<R> u = velocity(v.h, v.qx, v.qy);
     v.ux = u.x;
     v.uy = u.y;
     v.us = u.s;
-    v.c = 0.0;
-    if (SED && v.hc != 0.0) v.c = v.hc / (v.h + kEpsilon);
+    v.c = (R)0;
+    if (SED && v.hc != (R)0) v.c = rdiv(v.hc, v.h + (R)kEpsilon);
 }
 
 // Reconstructor.cu:146-194 along one axis: faces on the minus side (Fm) and
 // plus side (Fp) of the centre cell from its minus (l) and plus (r)
 // neighbours; returns the limited free-surface slope used by the
 // non-conservative source (Reconstructor.cu:197,274).
-template <bool SED>
-__device__ __forceinline__ double recons_axis(int lim, const Cell& ci, double hl, double Hl, double cl, double ulx,
-                                              double uly, double hr, double Hr, double cr, double urx, double ury,
-                                              double* Fm, double* Fp)
+template <typename R, bool SED>
+__device__ __forceinline__ R recons_axis(int lim, const Cell<R>& ci, R hl, R Hl, R cl, R ulx, R uly, R hr, R Hr, R cr,
+                                         R urx, R ury, R* Fm, R* Fp)
 {
-    const double hi = ci.h, Hi = ci.H;
-    double dl = hi - hl, dr = hr - hi;
-    double dh = limiter(lim, dl, dr);
-    if (0.5 * fabs(dh) > hi) dh = 2.0 * hi * (double)sgn(dh);
-    Fm[0] = hi - 0.5 * dh;
-    Fp[0] = hi + 0.5 * dh;
+    const R half = (R)0.5, zero = (R)0;
+    const R hi = ci.h, Hi = ci.H;
+    R dl = hi - hl, dr = hr - hi;
+    R dh = limiter(lim, dl, dr);
+    if (half * rabs(dh) > hi) dh = (R)2.0 * hi * (R)sgn(dh);
+    Fm[0] = hi - half * dh;
+    Fp[0] = hi + half * dh;
 
-    double zm = fmin(Hl, Hi);
-    dl = fmax(hi - Hi + zm, 0.0) - fmax(hl - Hl + zm, 0.0);
-    zm = fmin(Hr, Hi);
-    dr = fmax(hr - Hr + zm, 0.0) - fmax(hi - Hi + zm, 0.0);
-    double deta = limiter(lim, dl, dr);
-    double dz = dh - deta;
-    Fm[3] = Hi - 0.5 * dz;
-    Fp[3] = Hi + 0.5 * dz;
+    R zm = rmin(Hl, Hi);
+    dl = rmax(hi - Hi + zm, zero) - rmax(hl - Hl + zm, zero);
+    zm = rmin(Hr, Hi);
+    dr = rmax(hr - Hr + zm, zero) - rmax(hi - Hi + zm, zero);
+    R deta = limiter(lim, dl, dr);
+    R dz = dh - deta;
+    Fm[3] = Hi - half * dz;
+    Fp[3] = Hi + half * dz;
 
-    double du = limiter(lim, ci.ux - ulx, urx - ci.ux);
-    double dq = hi * du + ci.ux * dh;
-    Fm[1] = ci.qx - 0.5 * dq;
-    Fp[1] = ci.qx + 0.5 * dq;
+    R du = limiter(lim, ci.ux - ulx, urx - ci.ux);
+    R dq = hi * du + ci.ux * dh;
+    Fm[1] = ci.qx - half * dq;
+    Fp[1] = ci.qx + half * dq;
 
     du = limiter(lim, ci.uy - uly, ury - ci.uy);
     dq = hi * du + ci.uy * dh;
-    Fm[2] = ci.qy - 0.5 * dq;
-    Fp[2] = ci.qy + 0.5 * dq;
+    Fm[2] = ci.qy - half * dq;
+    Fp[2] = ci.qy + half * dq;
 
     if (SED) {
         du = limiter(lim, ci.c - cl, cr - ci.c);
-        Fm[4] = (ci.c - 0.5 * du) * Fm[0];
-        Fp[4] = (ci.c + 0.5 * du) * Fp[0];
+        Fm[4] = (ci.c - half * du) * Fm[0];
+        Fp[4] = (ci.c + half * du) * Fp[0];
     } else {
-        Fm[4] = Fp[4] = 0.0;
+        Fm[4] = Fp[4] = zero;
     }
     return deta;
 }
 
-template <bool SED>
-__device__ __forceinline__ void filter_face(double* F)
+template <typename R, bool SED>
+__device__ __forceinline__ void filter_face(R* F)
 {
-    state_filter<SED>(F[0], F[1], F[2], F[4]);
+    state_filter<R, SED>(F[0], F[1], F[2], F[4]);
 }
 
-__device__ __forceinline__ void cell_face(const Cell& c, double* F)
+template <typename R>
+__device__ __forceinline__ void cell_face(const Cell<R>& c, R* F)
 {
     F[0] = c.h;
     F[1] = c.qx;
@@ -143,14 +193,14 @@ __device__ __forceinline__ void cell_face(const Cell& c, double* F)
 
 // Riemann problem on one edge with left/right faces in the unrotated frame;
 // Dm, Dp come back in the unrotated frame (VolumeSolver.cu:19-170).
-template <bool SED>
-__device__ __forceinline__ void edge_flux(const DevParams& P, int kind, bool mobile, const double* Lf,
-                                          const double* Rf, double* Dm, double* Dp)
+template <typename R, bool SED>
+__device__ __forceinline__ void edge_flux(const DevParams& P, int kind, bool mobile, const R* Lf, const R* Rf, R* Dm,
+                                          R* Dp)
 {
-    double qnl, qtl, qnr, qtr, dm[5], dp[5];
+    R qnl, qtl, qnr, qtr, dm[5], dp[5];
     to_normal(kind, Lf[1], Lf[2], qnl, qtl);
     to_normal(kind, Rf[1], Rf[2], qnr, qtr);
-    riemann<SED>(P, mobile, Lf[0], Lf[3], qnl, qtl, Lf[4], Rf[0], Rf[3], qnr, qtr, Rf[4], dm, dp);
+    riemann<R, SED>(P, mobile, Lf[0], Lf[3], qnl, qtl, Lf[4], Rf[0], Rf[3], qnr, qtr, Rf[4], dm, dp);
     Dm[0] = dm[0];
     Dp[0] = dp[0];
     from_normal(kind, dm[1], dm[2], Dm[1], Dm[2]);
@@ -161,13 +211,14 @@ __device__ __forceinline__ void edge_flux(const DevParams& P, int kind, bool mob
         Dp[3] = dp[3];
         Dp[4] = dp[4];
     } else {
-        Dm[3] = Dm[4] = Dp[3] = Dp[4] = 0.0;
+        Dm[3] = Dm[4] = Dp[3] = Dp[4] = (R)0;
     }
 }
 
-template <typename S, bool SED>
+template <typename R, typename S, bool SED>
 __global__ __launch_bounds__(64) void stage_kernel(StageArgs a)
 {
+    constexpr bool COMP = sizeof(R) == 4;   // FP32 solver: compensated h, H, hc
     // Run constants are read through a pointer: fields used only in rare
     // branches are then loaded there instead of pinning scalar registers.
     const DevParams& P = *a.P;
@@ -179,7 +230,7 @@ __global__ __launch_bounds__(64) void stage_kernel(StageArgs a)
     const double ct = C->t;
     if (!(ct < C->stop_time)) return;
     const long long step = C->step;
-    const double dt = finalize_dt(P, C->dt_bits[step & 1]);
+    const double dtd = finalize_dt(P, C->dt_bits[step & 1]);
 
     const int npx = P.npx, npy = P.npy;
     const long long nv = P.nvol;
@@ -204,52 +255,54 @@ __global__ __launch_bounds__(64) void stage_kernel(StageArgs a)
     const bool need_fix = SED && P.bedload;
 
     // Per-step constants, in the reference's evaluation order.
-    const double invx = dt * P.inv_dx, invy = dt * P.inv_dy;  // main_TSUNER.cu:182
-    const double dtaxG = invx * kG, dtayG = invy * kG;        // Reconstructor.cu:197,274
-    const double dtFR = dt * P.FR;                             // Reconstructor.cu:45
-    const double dtGnn = dt * kG * P.manning_n * P.manning_n;
+    const double invxd = dtd * P.inv_dx, invyd = dtd * P.inv_dy;  // main_TSUNER.cu:182
+    const R dt = (R)dtd;
+    const R invx = (R)invxd, invy = (R)invyd;
+    const R dtaxG = (R)(invxd * kG), dtayG = (R)(invyd * kG);   // Reconstructor.cu:197,274
+    const R dtFR = (R)(dtd * P.FR);                               // Reconstructor.cu:45
 
     auto row_off = [&](int r) -> long long { return (long long)map_index(r, npy, pery) * npx + xc; };
     auto fix_at = [&](int r) -> int { return need_fix ? (int)fix[row_off(r)] : 0; };
 
     // ---- prologue: faces of row y0 and the edge y0 - 1/2 -----------------
-    Cell Rm, R0, R1;
-    load_cell<S, SED>(in, Hin, nv, row_off(y0 - 1), Rm);
-    load_cell<S, SED>(in, Hin, nv, row_off(y0), R0);
-    load_cell<S, SED>(in, Hin, nv, row_off(y0 + 1), R1);
-    derive_cell<SED>(Rm);
-    derive_cell<SED>(R0);
-    derive_cell<SED>(R1);
+    Cell<R> Rm, R0, R1;
+    load_cell<R, S, SED>(in, Hin, nv, row_off(y0 - 1), Rm);
+    load_cell<R, S, SED>(in, Hin, nv, row_off(y0), R0);
+    load_cell<R, S, SED>(in, Hin, nv, row_off(y0 + 1), R1);
+    derive_cell<R, SED>(Rm);
+    derive_cell<R, SED>(R0);
+    derive_cell<R, SED>(R1);
     int fx0 = fix_at(y0), fx1;
 
-    double D0[5], U0[5], FD0[5];
-    double detay0 = 0.0;
+    R D0[5], U0[5], FD0[5];
+    R detay0 = (R)0;
     if (second) {
-        detay0 = recons_axis<SED>(lim, R0, Rm.h, Rm.H, Rm.c, Rm.ux, Rm.uy, R1.h, R1.H, R1.c, R1.ux, R1.uy, D0, U0);
-        filter_face<SED>(D0);
-        filter_face<SED>(U0);
+        detay0 = recons_axis<R, SED>(lim, R0, Rm.h, Rm.H, Rm.c, Rm.ux, Rm.uy, R1.h, R1.H, R1.c, R1.ux, R1.uy, D0, U0);
+        filter_face<R, SED>(D0);
+        filter_face<R, SED>(U0);
     } else {
         cell_face(R0, D0);
         cell_face(R0, U0);
     }
     if (y0 == 0 && !pery) {
-        double G5[5], dp[5];
+        R G5[5], dp[5];
         ghost_state(P, 2, D0, G5, (long long)xc, 2, ct);
-        edge_flux<SED>(P, N_MY, fx0 != 0, D0, G5, FD0, dp);
+        edge_flux<R, SED>(P, N_MY, fx0 != 0, D0, G5, FD0, dp);
     } else {
-        double Um[5], Dd[5], dm[5];
+        R Um[5], Dd[5], dm[5];
         if (second) {
-            Cell Rmm;
-            load_cell<S, SED>(in, Hin, nv, row_off(y0 - 2), Rmm);
-            derive_cell<SED>(Rmm);
-            recons_axis<SED>(lim, Rm, Rmm.h, Rmm.H, Rmm.c, Rmm.ux, Rmm.uy, R0.h, R0.H, R0.c, R0.ux, R0.uy, Dd, Um);
-            filter_face<SED>(Um);
+            Cell<R> Rmm;
+            load_cell<R, S, SED>(in, Hin, nv, row_off(y0 - 2), Rmm);
+            derive_cell<R, SED>(Rmm);
+            recons_axis<R, SED>(lim, Rm, Rmm.h, Rmm.H, Rmm.c, Rmm.ux, Rmm.uy, R0.h, R0.H, R0.c, R0.ux, R0.uy, Dd,
+                                Um);
+            filter_face<R, SED>(Um);
         } else {
             cell_face(Rm, Um);
         }
         int fxm = fix_at(y0 - 1);
         int fixed = (y0 == 0) ? fxm : fxm * fx0;   // periodic wrap edge: VolumeSolver.cu:386
-        edge_flux<SED>(P, N_PY, fixed != 0, Um, D0, dm, FD0);
+        edge_flux<R, SED>(P, N_PY, fixed != 0, Um, D0, dm, FD0);
     }
 
     double dtmin = __longlong_as_double((long long)kInfBits);
@@ -257,42 +310,59 @@ __global__ __launch_bounds__(64) void stage_kernel(StageArgs a)
     // ---- march over the rows of the tile -----------------------------------
     for (int r = y0; r < y1; ++r) {
         // next row (r + 2): issued first, consumed after the x work below
-        Cell R2;
-        load_cell<S, SED>(in, Hin, nv, row_off(r + 2), R2);
+        Cell<R> R2;
+        load_cell<R, S, SED>(in, Hin, nv, row_off(r + 2), R2);
         fx1 = fix_at(r + 1);
-        double v0[5];
+        R v0[5];
+        R lo_in[3] = {(R)0, (R)0, (R)0}, lo_v0[3] = {(R)0, (R)0, (R)0};
         const long long pos = (long long)r * npx + xc;
-        if (a.k != 0 && out_lane) {
-            v0[0] = (double)v0p[pos];
-            v0[1] = (double)v0p[nv + pos];
-            v0[2] = (double)v0p[2 * nv + pos];
-            v0[3] = SED ? (double)v0p[3 * nv + pos] : R0.H;
-            v0[4] = SED ? (double)v0p[4 * nv + pos] : 0.0;
+        if (out_lane) {
+            if (a.k != 0) {
+                v0[0] = (R)v0p[pos];
+                v0[1] = (R)v0p[nv + pos];
+                v0[2] = (R)v0p[2 * nv + pos];
+                v0[3] = SED ? (R)v0p[3 * nv + pos] : R0.H;
+                v0[4] = SED ? (R)v0p[4 * nv + pos] : (R)0;
+            }
+            if (COMP) {
+                lo_in[0] = (R)in[5 * nv + pos];
+                if (SED) {
+                    lo_in[1] = (R)in[6 * nv + pos];
+                    lo_in[2] = (R)in[7 * nv + pos];
+                }
+                if (a.k != 0) {
+                    lo_v0[0] = (R)v0p[5 * nv + pos];
+                    if (SED) {
+                        lo_v0[1] = (R)v0p[6 * nv + pos];
+                        lo_v0[2] = (R)v0p[7 * nv + pos];
+                    }
+                }
+            }
         }
 
         // x faces of row r (Reconstructor.cu:120-197)
-        double hl = up1(R0.h), Hl = up1(R0.H), ulx = up1(R0.ux), uly = up1(R0.uy);
-        double hr = dn1(R0.h), Hr = dn1(R0.H), urx = dn1(R0.ux), ury = dn1(R0.uy);
-        double cl = 0.0, cr = 0.0;
+        R hl = up1(R0.h), Hl = up1(R0.H), ulx = up1(R0.ux), uly = up1(R0.uy);
+        R hr = dn1(R0.h), Hr = dn1(R0.H), urx = dn1(R0.ux), ury = dn1(R0.uy);
+        R cl = (R)0, cr = (R)0;
         if (SED) {
             cl = up1(R0.c);
             cr = dn1(R0.c);
         }
-        double Lf[5], Rf[5], detax = 0.0;
+        R Lf[5], Rf[5], detax = (R)0;
         if (second) {
-            detax = recons_axis<SED>(lim, R0, hl, Hl, cl, ulx, uly, hr, Hr, cr, urx, ury, Lf, Rf);
-            filter_face<SED>(Lf);
-            filter_face<SED>(Rf);
+            detax = recons_axis<R, SED>(lim, R0, hl, Hl, cl, ulx, uly, hr, Hr, cr, urx, ury, Lf, Rf);
+            filter_face<R, SED>(Lf);
+            filter_face<R, SED>(Rf);
         } else {
             cell_face(R0, Lf);
             cell_face(R0, Rf);
         }
 
         // x edge at x - 1/2 (VolumeSolver.cu:227-317)
-        double Rl[5];
+        R Rl[5];
         for (int f = 0; f < 5; f++) Rl[f] = up1(Rf[f]);
         int fxl = need_fix ? up1i(fx0) : 0;
-        double El[5], Er[5];
+        R El[5], Er[5];
         int kind = N_PX;
         int fixed;
         if (x == 0 && !perx) {
@@ -311,9 +381,9 @@ __global__ __launch_bounds__(64) void stage_kernel(StageArgs a)
             }
             fixed = (perx && (x == 0 || x == npx)) ? fxl : fxl * fx0;
         }
-        double Dmx[5], Dpx[5];
-        edge_flux<SED>(P, kind, fixed != 0, El, Er, Dmx, Dpx);
-        double FL[5], FR[5];
+        R Dmx[5], Dpx[5];
+        edge_flux<R, SED>(P, kind, fixed != 0, El, Er, Dmx, Dpx);
+        R FL[5], FR[5];
         const bool left_wall = (x == 0 && !perx);
         for (int f = 0; f < 5; f++) {
             FL[f] = left_wall ? Dmx[f] : Dpx[f];
@@ -321,24 +391,25 @@ __global__ __launch_bounds__(64) void stage_kernel(StageArgs a)
         }
 
         // y faces of row r + 1 and the edge r + 1/2
-        derive_cell<SED>(R2);
-        double D1[5], U1[5], detay1 = 0.0;
+        derive_cell<R, SED>(R2);
+        R D1[5], U1[5], detay1 = (R)0;
         if (second) {
-            detay1 = recons_axis<SED>(lim, R1, R0.h, R0.H, R0.c, R0.ux, R0.uy, R2.h, R2.H, R2.c, R2.ux, R2.uy, D1, U1);
-            filter_face<SED>(D1);
-            filter_face<SED>(U1);
+            detay1 = recons_axis<R, SED>(lim, R1, R0.h, R0.H, R0.c, R0.ux, R0.uy, R2.h, R2.H, R2.c, R2.ux, R2.uy, D1,
+                                         U1);
+            filter_face<R, SED>(D1);
+            filter_face<R, SED>(U1);
         } else {
             cell_face(R1, D1);
             cell_face(R1, U1);
         }
-        double FU0[5], FD1[5];
+        R FU0[5], FD1[5];
         if (r + 1 == npy && !pery) {
-            double G5[5], dp[5];
+            R G5[5], dp[5];
             ghost_state(P, 3, U0, G5, (long long)npx + npy + xc, 2, ct);
-            edge_flux<SED>(P, N_PY, fx0 != 0, U0, G5, FU0, dp);
+            edge_flux<R, SED>(P, N_PY, fx0 != 0, U0, G5, FU0, dp);
         } else {
             int fy = (r + 1 == npy) ? fx0 : fx0 * fx1;
-            edge_flux<SED>(P, N_PY, fy != 0, U0, D1, FU0, FD1);
+            edge_flux<R, SED>(P, N_PY, fy != 0, U0, D1, FU0, FD1);
         }
 
         // cell update of row r (Reconstructor.cu:197,274,353-354;
@@ -350,13 +421,15 @@ __global__ __launch_bounds__(64) void stage_kernel(StageArgs a)
                 v0[2] = R0.qy;
                 v0[3] = R0.H;
                 v0[4] = R0.hc;
+                if (COMP)
+                    for (int j = 0; j < 3; j++) lo_v0[j] = lo_in[j];
             }
-            double k1 = R0.qx, k2 = R0.qy, k3 = R0.H, k4 = R0.hc;
+            R k1 = R0.qx, k2 = R0.qy;
             if (second) {
                 k1 -= dtaxG * R0.h * detax;
                 k2 -= dtayG * R0.h * detay0;
             }
-            Vel un;
+            Vel<R> un;
             if (a.k == 0) {   // U^n is this stage's input: its velocity is known
                 un.x = R0.ux;
                 un.y = R0.uy;
@@ -364,56 +437,138 @@ __global__ __launch_bounds__(64) void stage_kernel(StageArgs a)
             } else {
                 un = velocity(v0[0], v0[1], v0[2]);
             }
-            const double modu = speed(un);
-            double cf;
+            const R modu = speed(un);
+            R cf;
             if (P.friction_law == FRIC_MANNING) {
-                double hp = v0[0] + kEpsilon;
-                cf = dtGnn * modu / (hp * cbrt(hp));
+                R hp = v0[0] + (R)kEpsilon;
+                R dtGnn = (R)(dtd * kG * P.manning_n * P.manning_n);
+                if constexpr (COMP)
+                    cf = dtGnn * modu / (hp * cbrtf(hp));
+                else
+                    cf = dtGnn * modu / (hp * cbrt(hp));
             } else {
-                cf = dtFR * modu / (v0[0] + kEpsilon);
+                cf = rdiv(dtFR * modu, v0[0] + (R)kEpsilon);
             }
-            k1 /= (1.0 + cf);
-            k2 /= (1.0 + cf);
-            if (SED && P.suspension) {
-                double phi = erosion_rate(P, v0[0], modu, v0[4]);
-                k3 -= dt * phi;
-                k4 += dt * phi;
+            if constexpr (COMP) {
+                R rc = __builtin_amdgcn_rcpf((R)1 + cf);
+                k1 *= rc;
+                k2 *= rc;
+            } else {
+                k1 /= (1.0 + cf);
+                k2 /= (1.0 + cf);
             }
-            double W[5];
-            W[0] = R0.h - (invx * (FL[0] + FR[0]) + invy * (FD0[0] + FU0[0]));
+            R phi = (R)0;
+            if (SED && P.suspension) phi = erosion_rate(P, v0[0], modu, v0[4]);
+
+            R W[5];
             W[1] = k1 - (invx * (FL[1] + FR[1]) + invy * (FD0[1] + FU0[1]));
             W[2] = k2 - (invx * (FL[2] + FR[2]) + invy * (FD0[2] + FU0[2]));
-            if (SED) {
-                W[3] = k3 - (invx * (FL[3] + FR[3]) + invy * (FD0[3] + FU0[3]));
-                W[4] = k4 - (invx * (FL[4] + FR[4]) + invy * (FD0[4] + FU0[4]));
+            if constexpr (!COMP) {
+                R k3 = R0.H, k4 = R0.hc;
+                if (SED && P.suspension) {
+                    k3 -= dt * phi;
+                    k4 += dt * phi;
+                }
+                W[0] = R0.h - (invx * (FL[0] + FR[0]) + invy * (FD0[0] + FU0[0]));
+                if (SED) {
+                    W[3] = k3 - (invx * (FL[3] + FR[3]) + invy * (FD0[3] + FU0[3]));
+                    W[4] = k4 - (invx * (FL[4] + FR[4]) + invy * (FD0[4] + FU0[4]));
+                } else {
+                    W[3] = R0.H;
+                    W[4] = (R)0;
+                }
+                if (a.k == 1) {
+                    const int nf = SED ? 5 : 3;
+                    for (int f = 0; f < nf; f++) W[f] = 0.5 * v0[f] + 0.5 * W[f];
+                }
+                state_filter<R, SED>(W[0], W[1], W[2], W[4]);
+                out[pos] = (S)W[0];
+                out[nv + pos] = (S)W[1];
+                out[2 * nv + pos] = (S)W[2];
+                if (SED) {
+                    out[3 * nv + pos] = (S)W[3];
+                    out[4 * nv + pos] = (S)W[4];
+                }
             } else {
-                W[3] = R0.H;
-                W[4] = 0.0;
-            }
-            if (a.k == 1) {
-                const int nf = SED ? 5 : 3;
-                for (int f = 0; f < nf; f++) W[f] = 0.5 * v0[f] + 0.5 * W[f];
-            }
-            state_filter<SED>(W[0], W[1], W[2], W[4]);
-            out[pos] = (S)W[0];
-            out[nv + pos] = (S)W[1];
-            out[2 * nv + pos] = (S)W[2];
-            if (SED) {
-                out[3 * nv + pos] = (S)W[3];
-                out[4 * nv + pos] = (S)W[4];
+                // Compensated accumulation: every flux term enters its pair
+                // exactly; the same product leaves the neighbour across the
+                // edge (Dp = -Dm), so exchanges cancel.
+                const int nm = SED ? 3 : 1;
+                const int plane[3] = {0, 3, 4};
+                Pair m[3];
+                for (int j = 0; j < nm; j++) {
+                    const int f = plane[j];
+                    m[j].hi = (j == 0) ? R0.h : (j == 1 ? R0.H : R0.hc);
+                    m[j].lo = lo_in[j];
+                    pacc(m[j], -(invx * FL[f]));
+                    pacc(m[j], -(invx * FR[f]));
+                    pacc(m[j], -(invy * FD0[f]));
+                    pacc(m[j], -(invy * FU0[f]));
+                }
+                if (SED && P.suspension) {
+                    const R ex = dt * phi;
+                    pacc(m[1], -ex);
+                    pacc(m[2], ex);
+                }
+                for (int j = 0; j < nm; j++) m[j] = pnorm(m[j]);
+                if (a.k == 1) {
+                    for (int j = 0; j < nm; j++) {
+                        Pair b0;
+                        b0.hi = v0[plane[j]];
+                        b0.lo = lo_v0[j];
+                        m[j] = pavg(b0, m[j]);
+                    }
+                    W[1] = (R)0.5 * v0[1] + (R)0.5 * W[1];
+                    W[2] = (R)0.5 * v0[2] + (R)0.5 * W[2];
+                }
+                // state filter (SedFlux.cuh:118-132) on the pair values
+                if (m[0].hi < (R)0 || (m[0].hi == (R)0 && m[0].lo <= (R)0)) m[0].hi = m[0].lo = (R)0;
+                const R h = m[0].hi;
+                if (h < kNearDry) {
+                    R hh = h, hcv = SED ? m[2].hi : (R)0;
+                    state_filter<R, SED>(hh, W[1], W[2], hcv);
+                    if (SED && hcv == (R)0) m[2].hi = m[2].lo = (R)0;
+                } else {
+                    // wet: q = h * (q / h) is q to float rounding; only the cap acts
+                    R rh = __builtin_amdgcn_rcpf(h);
+                    R ux = W[1] * rh, uy = W[2] * rh;
+                    R s = ux * ux + uy * uy;
+                    if (s > (R)100.0f) {
+                        R kk = (R)kVmax * __builtin_amdgcn_rsqf(s);
+                        W[1] = h * (ux * kk);
+                        W[2] = h * (uy * kk);
+                    }
+                }
+                if (SED && m[2].hi < (R)0) m[2].hi = m[2].lo = (R)0;
+                W[0] = m[0].hi;
+                out[pos] = (S)m[0].hi;
+                out[5 * nv + pos] = (S)m[0].lo;
+                out[nv + pos] = (S)W[1];
+                out[2 * nv + pos] = (S)W[2];
+                if (SED) {
+                    W[3] = m[1].hi;
+                    W[4] = m[2].hi;
+                    out[3 * nv + pos] = (S)m[1].hi;
+                    out[6 * nv + pos] = (S)m[1].lo;
+                    out[4 * nv + pos] = (S)m[2].hi;
+                    out[7 * nv + pos] = (S)m[2].lo;
+                } else {
+                    W[3] = R0.H;
+                    W[4] = (R)0;
+                }
             }
             if (a.last) {
                 // the next dt is evaluated on the stored state
-                double h = (double)(S)W[0], qx = (double)(S)W[1], qy = (double)(S)W[2];
-                dtmin = fmin(dtmin, cell_dt(P, h, qx, qy));
+                const R h = W[0];
+                dtmin = fmin(dtmin, (double)cell_dt(P, h, W[1], W[2]));
                 if (a.ext) {
-                    double Hs = (double)(S)W[3];
+                    double hd = (double)h, Hs = (double)W[3];
                     double* hmax = a.ext;
                     double* bmax = a.ext + nv;
                     double* emax = a.ext + 2 * nv;
-                    if (h > P.epsilon_h) {
-                        hmax[pos] = fmax(hmax[pos], h);
-                        emax[pos] = fmax(emax[pos], h - Hs);
+                    if (hd > P.epsilon_h) {
+                        hmax[pos] = fmax(hmax[pos], hd);
+                        emax[pos] = fmax(emax[pos], hd - Hs);
                     }
                     bmax[pos] = fmax(bmax[pos], Hs);
                 }
@@ -439,13 +594,13 @@ __global__ __launch_bounds__(64) void stage_kernel(StageArgs a)
 }
 
 // GPU/main_TSUNER.cu:147-148: dt of the initial state into dt_bits[step & 1].
-template <typename S>
+template <typename R, typename S>
 __global__ __launch_bounds__(256) void init_dt_kernel(const S* V, DevParams P, Ctrl* C)
 {
     const long long nv = P.nvol;
     double m = __longlong_as_double((long long)kInfBits);
     for (long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x; i < nv; i += (long long)gridDim.x * blockDim.x)
-        m = fmin(m, cell_dt(P, (double)V[i], (double)V[nv + i], (double)V[2 * nv + i]));
+        m = fmin(m, (double)cell_dt(P, (R)V[i], (R)V[nv + i], (R)V[2 * nv + i]));
     m = wave_min(m);
     if ((threadIdx.x & 63) == 0 && m < __longlong_as_double((long long)kInfBits))
         atomicMin(&C->dt_bits[C->step & 1], (unsigned long long)__double_as_longlong(m));
@@ -482,8 +637,9 @@ __global__ void fill_kernel(double* p, long long n, double v)
         p[i] = v;
 }
 
-// Deterministic two-pass FP64 sums of h and of (H + hc).
-template <typename S>
+// Deterministic two-pass FP64 sums of h and of (H + hc); the FP32 solver
+// adds the lo planes (planes 5, 6, 7).
+template <typename S, bool COMP>
 __global__ __launch_bounds__(256) void mass_partial_kernel(const S* V, long long nv, double* partial)
 {
     __shared__ double sh[2][4];
@@ -491,6 +647,10 @@ __global__ __launch_bounds__(256) void mass_partial_kernel(const S* V, long long
     for (long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x; i < nv; i += (long long)gridDim.x * blockDim.x) {
         a += (double)V[i];
         b += (double)V[3 * nv + i] + (double)V[4 * nv + i];
+        if (COMP) {
+            a += (double)V[5 * nv + i];
+            b += (double)V[6 * nv + i] + (double)V[7 * nv + i];
+        }
     }
     for (int o = 32; o > 0; o >>= 1) {
         a += __shfl_xor(a, o, 64);
@@ -530,7 +690,7 @@ __global__ void kat_edge_kernel(DevParams P, int n, const double* left, const do
         L[f] = left[5 * i + f];
         R[f] = right[5 * i + f];
     }
-    edge_flux<true>(P, kind[i], fixed[i] != 0 && P.bedload, L, R, dm, dp);
+    edge_flux<double, true>(P, kind[i], fixed[i] != 0 && P.bedload, L, R, dm, dp);
     for (int f = 0; f < 5; f++) {
         Dm[5 * i + f] = dm[f];
         Dp[5 * i + f] = dp[f];
@@ -550,7 +710,7 @@ __global__ void kat_point_kernel(int what, int law, int n, const double* a, cons
         o1[i] = limiter(law, a[i], b[i]);
         break;
     case 1: {
-        Vel u = velocity(a[i], b[i], c[i]);
+        Vel<double> u = velocity(a[i], b[i], c[i]);
         o1[i] = u.x;
         o2[i] = u.y;
         o3[i] = speed(u);
@@ -558,7 +718,7 @@ __global__ void kat_point_kernel(int what, int law, int n, const double* a, cons
     }
     case 2: {
         double h = a[i], qx = b[i], qy = c[i], hc = d[i];
-        state_filter<true>(h, qx, qy, hc);
+        state_filter<double, true>(h, qx, qy, hc);
         o1[i] = h;
         o2[i] = qx;
         o3[i] = qy;
@@ -569,12 +729,12 @@ __global__ void kat_point_kernel(int what, int law, int n, const double* a, cons
         o1[i] = cell_dt(P, a[i], b[i], c[i]);
         break;
     case 4: {
-        Vel u = velocity(a[i], b[i], c[i]);
+        Vel<double> u = velocity(a[i], b[i], c[i]);
         o1[i] = erosion_rate(P, a[i], speed(u), d[i]);
         break;
     }
     case 5: {
-        Vel u = velocity(1.0, b[i], c[i]);
+        Vel<double> u;
         u.x = b[i];
         u.y = c[i];
         u.s = u.x * u.x + u.y * u.y;
@@ -591,14 +751,14 @@ hipError_t launch_stage(const StageArgs& a, bool sed, int precision, hipStream_t
     dim3 grid(a.ntiles), block(64);
     if (precision == 0) {
         if (sed)
-            hipLaunchKernelGGL((stage_kernel<double, true>), grid, block, 0, s, a);
+            hipLaunchKernelGGL((stage_kernel<double, double, true>), grid, block, 0, s, a);
         else
-            hipLaunchKernelGGL((stage_kernel<double, false>), grid, block, 0, s, a);
+            hipLaunchKernelGGL((stage_kernel<double, double, false>), grid, block, 0, s, a);
     } else {
         if (sed)
-            hipLaunchKernelGGL((stage_kernel<float, true>), grid, block, 0, s, a);
+            hipLaunchKernelGGL((stage_kernel<float, float, true>), grid, block, 0, s, a);
         else
-            hipLaunchKernelGGL((stage_kernel<float, false>), grid, block, 0, s, a);
+            hipLaunchKernelGGL((stage_kernel<float, float, false>), grid, block, 0, s, a);
     }
     return hipGetLastError();
 }
@@ -608,9 +768,9 @@ hipError_t launch_init_dt(const void* V, const DevParams& P, Ctrl* ctrl, int pre
     long long nb = (P.nvol + 255) / 256;
     int grid = (int)(nb < 4096 ? (nb > 0 ? nb : 1) : 4096);
     if (precision == 0)
-        hipLaunchKernelGGL(init_dt_kernel<double>, dim3(grid), dim3(256), 0, s, (const double*)V, P, ctrl);
+        hipLaunchKernelGGL((init_dt_kernel<double, double>), dim3(grid), dim3(256), 0, s, (const double*)V, P, ctrl);
     else
-        hipLaunchKernelGGL(init_dt_kernel<float>, dim3(grid), dim3(256), 0, s, (const float*)V, P, ctrl);
+        hipLaunchKernelGGL((init_dt_kernel<float, float>), dim3(grid), dim3(256), 0, s, (const float*)V, P, ctrl);
     return hipGetLastError();
 }
 
@@ -651,9 +811,11 @@ hipError_t launch_mass(const void* V, long long nvol, int precision, double* par
                        hipStream_t s)
 {
     if (precision == 0)
-        hipLaunchKernelGGL(mass_partial_kernel<double>, dim3(nblocks), dim3(256), 0, s, (const double*)V, nvol, partial);
+        hipLaunchKernelGGL((mass_partial_kernel<double, false>), dim3(nblocks), dim3(256), 0, s, (const double*)V,
+                           nvol, partial);
     else
-        hipLaunchKernelGGL(mass_partial_kernel<float>, dim3(nblocks), dim3(256), 0, s, (const float*)V, nvol, partial);
+        hipLaunchKernelGGL((mass_partial_kernel<float, true>), dim3(nblocks), dim3(256), 0, s, (const float*)V, nvol,
+                           partial);
     hipLaunchKernelGGL(mass_final_kernel, dim3(1), dim3(64), 0, s, partial, nblocks, out);
     return hipGetLastError();
 }
