@@ -295,8 +295,12 @@ def test_fp32_lake_at_rest_exact():
     W = g.state()
     np.testing.assert_array_equal(W[1], 0.0)
     np.testing.assert_array_equal(W[2], 0.0)
-    np.testing.assert_array_equal(W[0], sc.V[0])
-    np.testing.assert_array_equal(W[3], sc.V[3])
+    # the FP32 state holds hi + lo float pairs (48 significant bits)
+    def pair(v):
+        hi = v.astype(np.float32)
+        return hi.astype(np.float64) + (v - hi.astype(np.float64)).astype(np.float32).astype(np.float64)
+    np.testing.assert_array_equal(W[0], pair(sc.V[0]))
+    np.testing.assert_array_equal(W[3], pair(sc.V[3]))
 
 
 def test_fp32_compensated_conservation():
