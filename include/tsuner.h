@@ -160,6 +160,36 @@ TSU_API void *tsu_stream(tsu_state *st);
 /* Device pointer of the U^n planes (element size 8 or 4 by precision). */
 TSU_API void *tsu_device_state(tsu_state *st);
 
+/* ---- ensembles ------------------------------------------------------------
+ * n independent scenarios (grids of any size) advanced together: one launch
+ * per RK stage covers every member, each member keeps its own dt and time,
+ * and whole-ensemble steps replay from a captured HIP graph.  The reference
+ * runs one scenario per TSUNER() call (GPU/main_TSUNER.cu:21-222); members
+ * here follow the same arithmetic.  params and scheme are arrays of n; all
+ * members share the RK order; inflow series are not supported. */
+typedef struct tsu_ensemble tsu_ensemble;
+
+TSU_API int tsu_ensemble_create(tsu_ensemble **out, int n, const tsu_sim_params *params,
+                                const tsu_scheme *scheme, int precision, int device);
+TSU_API int tsu_ensemble_destroy(tsu_ensemble *e);
+TSU_API int tsu_ensemble_upload(tsu_ensemble *e, int member, const double *V, const int *fixH);
+TSU_API int tsu_ensemble_download(tsu_ensemble *e, int member, double *V);
+TSU_API int tsu_ensemble_init_dt(tsu_ensemble *e);
+/* Members stop (their steps become no-ops) after stop_step steps. */
+TSU_API int tsu_ensemble_set_stop_step(tsu_ensemble *e, long long stop_step);
+TSU_API int tsu_ensemble_step(tsu_ensemble *e, int nsteps);
+TSU_API int tsu_ensemble_synchronize(tsu_ensemble *e);
+TSU_API int tsu_ensemble_get_time(tsu_ensemble *e, int member, double *t, double *dt, long long *steps);
+TSU_API int tsu_ensemble_mass(tsu_ensemble *e, int member, double out[2]);
+/* Tile height per member (n ints), e.g. chosen by tsuner_amd.autotune. */
+TSU_API int tsu_ensemble_set_tile_rows(tsu_ensemble *e, const int *rows);
+TSU_API int tsu_ensemble_set_graph_chunk(tsu_ensemble *e, int chunk);
+/* restore == 0 saves every member's state and step control; 1 restores it
+ * (second-order schemes). */
+TSU_API int tsu_ensemble_snapshot(tsu_ensemble *e, int restore);
+/* hipEvent-timed wall time of nsteps eager whole-ensemble steps. */
+TSU_API int tsu_ensemble_time_steps(tsu_ensemble *e, int nsteps, double *ms);
+
 /* Achievable HBM bandwidth: times reps launches of a plain device copy of
  * `bytes` bytes and returns (read + write bytes) / mean launch time in GB/s. */
 TSU_API int tsu_bandwidth_probe(int device, long long bytes, int reps, double *gbs);

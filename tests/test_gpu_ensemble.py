@@ -1,0 +1,70 @@
+"""Ensembles: every member of a batched, uneven-size ensemble advances exactly
+as a single-grid run of the same scenario (and, in FP64, as the oracle)."""
+import numpy as np
+import pytest
+
+from tsuner_amd import scenarios as S
+from tsuner_amd.ensemble import Ensemble
+from tsuner_amd.solver import Solver
+
+pytestmark = pytest.mark.gpu
+
+
+def members():
+    out = [S.gaussian_source(n=n, dx=40.0, seed=seed) for n, seed in ((48, 1), (70, 2), (61, 3), (130, 4))]
+    sc = S.sediment_shelf(n=90, ny=45, dx=40.0, inflow=False, seed=5)
+    out.append(sc)
+    return out
+
+
+def test_fp64_members_equal_oracle_and_single(oracle):
+    scs = members()
+    ens = Ensemble(scs, precision="fp64", tile_rows=[8, 16, 5, 32, 3])
+    ens.step(12)
+    for i, sc in enumerate(scs):
+        print("member %d %s" % (i, sc.name), flush=True)
+        o = oracle.from_scenario(sc)
+        o.step(12)
+        W = ens.state(i)
+        assert np.array_equal(W, o.state()), "member %d differs from the oracle" % i
+        t, dt, n = ens.time(i)
+        assert (t, dt, n) == (o.t, o.dt, 12)
+
+
+def test_fp32_members_equal_single_solver():
+    scs = members()
+    ens = Ensemble(scs, precision="fp32")
+    ens.step(15)
+    for i, sc in enumerate(scs):
+        g = Solver.from_scenario(sc, precision="fp32", tile_rows=16)
+        g.step(15)
+        assert np.array_equal(ens.state(i), g.state()), "member %d" % i
+        assert ens.time(i) == g.time()
+
+
+def test_graph_replay_and_stop_step():
+    scs = members()
+    a = Ensemble(scs, precision="fp32")
+    b = Ensemble(scs, precision="fp32", graph_chunk=4)
+    b.set_stop_step(10)
+    a.step(10)
+    b.step(12)          # three replays of four steps; the last two steps are no-ops
+    for i in range(len(scs)):
+        assert b.time(i)[2] == 10
+        assert np.array_equal(a.state(i), b.state(i))
+        b.mass(i)
+
+
+def test_autotune_keeps_results(tmp_path, monkeypatch):
+    from tsuner_amd import autotune
+    monkeypatch.setenv("XDG_CACHE_HOME", str(tmp_path))
+    scs = members()
+    ref = Ensemble(scs, precision="fp32")
+    ref.step(6)
+    ens = Ensemble(scs, precision="fp32")
+    choice = autotune.tune_ensemble(ens, "fp32", candidates=(4, 16), steps=2, log=print)
+    assert set(choice) == {max(s.npx, s.npy) for s in scs}
+    ens.step(6)
+    for i in range(len(scs)):
+        assert np.array_equal(ref.state(i), ens.state(i))
+    assert (tmp_path / "tsuner_amd" / "tile_rows.json").exists()

@@ -97,6 +97,21 @@ _PROTOS = {
     "tsu_get_timing": (C.c_int, [C.c_void_p, _dp, C.POINTER(C.c_longlong), C.c_int]),
     "tsu_stream": (C.c_void_p, [C.c_void_p]),
     "tsu_device_state": (C.c_void_p, [C.c_void_p]),
+    "tsu_ensemble_create": (C.c_int, [C.POINTER(C.c_void_p), C.c_int, C.POINTER(SimParams), C.POINTER(Scheme),
+                                      C.c_int, C.c_int]),
+    "tsu_ensemble_destroy": (C.c_int, [C.c_void_p]),
+    "tsu_ensemble_upload": (C.c_int, [C.c_void_p, C.c_int, _dp, _ip]),
+    "tsu_ensemble_download": (C.c_int, [C.c_void_p, C.c_int, _dp]),
+    "tsu_ensemble_init_dt": (C.c_int, [C.c_void_p]),
+    "tsu_ensemble_set_stop_step": (C.c_int, [C.c_void_p, C.c_longlong]),
+    "tsu_ensemble_step": (C.c_int, [C.c_void_p, C.c_int]),
+    "tsu_ensemble_synchronize": (C.c_int, [C.c_void_p]),
+    "tsu_ensemble_get_time": (C.c_int, [C.c_void_p, C.c_int, _dp, _dp, C.POINTER(C.c_longlong)]),
+    "tsu_ensemble_mass": (C.c_int, [C.c_void_p, C.c_int, _dp]),
+    "tsu_ensemble_set_tile_rows": (C.c_int, [C.c_void_p, _ip]),
+    "tsu_ensemble_set_graph_chunk": (C.c_int, [C.c_void_p, C.c_int]),
+    "tsu_ensemble_snapshot": (C.c_int, [C.c_void_p, C.c_int]),
+    "tsu_ensemble_time_steps": (C.c_int, [C.c_void_p, C.c_int, _dp]),
     "tsu_bandwidth_probe": (C.c_int, [C.c_int, C.c_longlong, C.c_int, _dp]),
     "tsu_last_error": (C.c_char_p, []),
     "tsu_version": (C.c_char_p, []),

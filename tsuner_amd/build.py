@@ -19,7 +19,7 @@ BUILD = os.path.join(HERE, "_build")
 LIB = os.path.join(HERE, "libtsuner.so")
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 
-SOURCES = ["tsu_kernels.hip", "tsu_api.cpp", "tsu_run.cpp", "tsu_netcdf.cpp"]
+SOURCES = ["tsu_kernels.hip", "tsu_api.cpp", "tsu_run.cpp", "tsu_netcdf.cpp", "tsu_ensemble.cpp"]
 
 COMMON = ["-O3", "-fPIC", "-std=c++17", "-ffp-contract=off", "-fno-fast-math", "-fvisibility=hidden",
           "-Wall", "-Wno-unused-function", "-I" + CSRC, "-I" + os.path.join(os.path.dirname(HERE), "include")]

@@ -12,12 +12,14 @@
 #include <hip/hip_runtime.h>
 
 #include <chrono>
+#include <climits>
 #include <cmath>
 #include <cstdio>
 #include <cstring>
 #include <string>
 #include <vector>
 
+#include "tsu_host.h"
 #include "tsu_kernels.h"
 
 using tsu::Ctrl;
@@ -126,6 +128,10 @@ void make_dev_params(const tsu_sim_params* p, const tsu_scheme* s, DevParams* P)
     for (int i = 0; i < 4; i++) P->bc[i] = s->bc[i];
     P->ring = 2LL * p->npx + 2LL * p->npy;
 }
+
+int validate_scheme(const tsu_scheme* s) { return check_scheme(s); }
+
+int set_error(int code, const std::string& msg) { return fail(code, msg); }
 
 }  // namespace tsu
 
@@ -384,6 +390,7 @@ TSU_API int tsu_state_create(tsu_state** out, const tsu_sim_params* p, const tsu
     c.stop_time = INFINITY;
     c.dt_bits[0] = c.dt_bits[1] = kInfBits;
     c.step = 0;
+    c.stop_step = LLONG_MAX;
     e = hipMemcpy(st->ctrl, &c, sizeof(Ctrl), hipMemcpyHostToDevice);
     if (e != hipSuccess) {
         free_state(st);
@@ -436,6 +443,7 @@ TSU_API int tsu_state_upload(tsu_state* st, const double* V, const int* fixH)
     c.stop_time = INFINITY;
     c.dt_bits[0] = c.dt_bits[1] = kInfBits;
     c.step = 0;
+    c.stop_step = LLONG_MAX;
     HIPCHK(hipMemcpyAsync(st->ctrl, &c, sizeof(Ctrl), hipMemcpyHostToDevice, st->stream));
     if (st->ext) HIPCHK(tsu::launch_fill(st->ext, 3 * nv, -1e30, st->stream));
     HIPCHK(hipStreamSynchronize(st->stream));

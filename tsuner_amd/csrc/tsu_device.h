@@ -80,7 +80,8 @@ struct Ctrl {
     double stop_time;
     unsigned long long dt_bits[2];
     long long step;
-    long long pad[3];
+    long long stop_step;   // steps enqueued once step >= stop_step do nothing
+    long long pad[2];
 };
 
 // ---- primitive operations per compute type --------------------------------

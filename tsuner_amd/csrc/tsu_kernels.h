@@ -31,6 +31,10 @@ struct StageArgs {
 
 // precision: 0 = FP64 planes, 1 = FP32 planes (FP64 arithmetic).
 hipError_t launch_stage(const StageArgs& a, bool sed, int precision, hipStream_t s);
+// Ensemble: one launch over all tiles of all members (tiles[i] = member, tile).
+hipError_t launch_stage_batch(const StageArgs* scen, const int2* tiles, int ntiles, int k, int last, bool sed,
+                              int precision, hipStream_t s);
+hipError_t launch_advance_batch(Ctrl* ctrl, const DevParams* P, int n, hipStream_t s);
 hipError_t launch_init_dt(const void* V, const DevParams& P, Ctrl* ctrl, int precision, hipStream_t s);
 hipError_t launch_warmup(hipStream_t s);
 hipError_t launch_copy(const void* src, void* dst, long long bytes, hipStream_t s);

@@ -216,19 +216,18 @@ __device__ __forceinline__ void edge_flux(const DevParams& P, int kind, bool mob
 }
 
 template <typename R, typename S, bool SED>
-__global__ __launch_bounds__(64) void stage_kernel(StageArgs a)
+__device__ __forceinline__ void stage_body(const StageArgs& a, const int tile)
 {
     constexpr bool COMP = sizeof(R) == 4;   // FP32 solver: compensated h, H, hc
     // Run constants are read through a pointer: fields used only in rare
     // branches are then loaded there instead of pinning scalar registers.
     const DevParams& P = *a.P;
     const int lane = threadIdx.x;
-    const int tile = xcd_tile(blockIdx.x, gridDim.x);
     const int strip = tile % a.nstrips, chunk = tile / a.nstrips;
 
     Ctrl* C = a.ctrl;
     const double ct = C->t;
-    if (!(ct < C->stop_time)) return;
+    if (!(ct < C->stop_time) || C->step >= C->stop_step) return;
     const long long step = C->step;
     const double dtd = finalize_dt(P, C->dt_bits[step & 1]);
 
@@ -593,6 +592,26 @@ __global__ __launch_bounds__(64) void stage_kernel(StageArgs a)
     }
 }
 
+template <typename R, typename S, bool SED>
+__global__ __launch_bounds__(64) void stage_kernel(StageArgs a)
+{
+    stage_body<R, S, SED>(a, xcd_tile(blockIdx.x, gridDim.x));
+}
+
+// Ensemble: one launch covers every tile of every scenario.  tiles[b] =
+// (scenario, tile within the scenario); scen[s] holds that scenario's
+// buffers, constants and step-control block.
+template <typename R, typename S, bool SED>
+__global__ __launch_bounds__(64) void stage_kernel_batch(const StageArgs* __restrict__ scen,
+                                                         const int2* __restrict__ tiles, int k, int last)
+{
+    const int2 e = tiles[xcd_tile(blockIdx.x, gridDim.x)];
+    StageArgs a = scen[e.x];
+    a.k = k;
+    a.last = last;
+    stage_body<R, S, SED>(a, e.y);
+}
+
 // GPU/main_TSUNER.cu:147-148: dt of the initial state into dt_bits[step & 1].
 template <typename R, typename S>
 __global__ __launch_bounds__(256) void init_dt_kernel(const S* V, DevParams P, Ctrl* C)
@@ -608,16 +627,28 @@ __global__ __launch_bounds__(256) void init_dt_kernel(const S* V, DevParams P, C
 
 // GPU/main_TSUNER.cu:185: t += dt after the stages; the dt slot of the
 // finished step is re-armed for step + 2.
-__global__ void advance_kernel(Ctrl* C, DevParams P)
+__device__ __forceinline__ void advance_one(Ctrl* C, const DevParams& P)
 {
-    if (threadIdx.x != 0 || blockIdx.x != 0) return;
     const double t = C->t;
-    if (!(t < C->stop_time)) return;
     const long long s = C->step;
+    if (!(t < C->stop_time) || s >= C->stop_step) return;
     const double dt = finalize_dt(P, C->dt_bits[s & 1]);
     C->t = t + dt;
     C->dt_bits[s & 1] = kInfBits;
     C->step = s + 1;
+}
+
+__global__ void advance_kernel(Ctrl* C, DevParams P)
+{
+    if (threadIdx.x != 0 || blockIdx.x != 0) return;
+    advance_one(C, P);
+}
+
+// One thread per ensemble member.
+__global__ void advance_batch_kernel(Ctrl* C, const DevParams* __restrict__ P, int n)
+{
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) advance_one(C + i, P[i]);
 }
 
 __global__ void warmup_kernel() {}
@@ -760,6 +791,30 @@ hipError_t launch_stage(const StageArgs& a, bool sed, int precision, hipStream_t
         else
             hipLaunchKernelGGL((stage_kernel<float, float, false>), grid, block, 0, s, a);
     }
+    return hipGetLastError();
+}
+
+hipError_t launch_stage_batch(const StageArgs* scen, const int2* tiles, int ntiles, int k, int last, bool sed,
+                              int precision, hipStream_t s)
+{
+    dim3 grid(ntiles), block(64);
+    if (precision == 0) {
+        if (sed)
+            hipLaunchKernelGGL((stage_kernel_batch<double, double, true>), grid, block, 0, s, scen, tiles, k, last);
+        else
+            hipLaunchKernelGGL((stage_kernel_batch<double, double, false>), grid, block, 0, s, scen, tiles, k, last);
+    } else {
+        if (sed)
+            hipLaunchKernelGGL((stage_kernel_batch<float, float, true>), grid, block, 0, s, scen, tiles, k, last);
+        else
+            hipLaunchKernelGGL((stage_kernel_batch<float, float, false>), grid, block, 0, s, scen, tiles, k, last);
+    }
+    return hipGetLastError();
+}
+
+hipError_t launch_advance_batch(Ctrl* ctrl, const DevParams* P, int n, hipStream_t s)
+{
+    hipLaunchKernelGGL(advance_batch_kernel, dim3((n + 63) / 64), dim3(64), 0, s, ctrl, P, n);
     return hipGetLastError();
 }
 
