@@ -119,6 +119,14 @@ TSU_API int tsu_write_output(const char *output_prefix, const tsu_sim_params *pa
                              const double *times, const double *V_records, const double *hmax,
                              const double *bmax, const double *eta_max);
 
+/* Reads the reference's positional data.dat (data.dat:1-45, parsed as
+ * Problem_loader.cxx:264-335) into params with the derived fields of
+ * Problem_loader.cxx:337-342 and dx = (xmax-xmin)/npx, dy likewise.  prefix
+ * receives the output prefix; files the bathymetry, eta/CC, fixed-bed and
+ * tides file names separated by newlines. */
+TSU_API int tsu_parse_data_dat(const char *path, tsu_sim_params *params, char *prefix, int prefix_len,
+                               char *files, int files_len, double *sea_level_correction);
+
 /* ---- handle API --------------------------------------------------------- */
 
 typedef struct tsu_state tsu_state;

@@ -113,6 +113,8 @@ _PROTOS = {
     "tsu_ensemble_snapshot": (C.c_int, [C.c_void_p, C.c_int]),
     "tsu_ensemble_time_steps": (C.c_int, [C.c_void_p, C.c_int, _dp]),
     "tsu_bandwidth_probe": (C.c_int, [C.c_int, C.c_longlong, C.c_int, _dp]),
+    "tsu_parse_data_dat": (C.c_int, [C.c_char_p, C.POINTER(SimParams), C.c_char_p, C.c_int, C.c_char_p, C.c_int,
+                                     _dp]),
     "tsu_last_error": (C.c_char_p, []),
     "tsu_version": (C.c_char_p, []),
     "tsu_kat_edge_flux": (C.c_int, [C.POINTER(SimParams), C.POINTER(Scheme), C.c_int, _dp, _dp, _ip, _ip, _dp,

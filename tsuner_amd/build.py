@@ -19,7 +19,8 @@ BUILD = os.path.join(HERE, "_build")
 LIB = os.path.join(HERE, "libtsuner.so")
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 
-SOURCES = ["tsu_kernels.hip", "tsu_api.cpp", "tsu_run.cpp", "tsu_netcdf.cpp", "tsu_ensemble.cpp"]
+SOURCES = ["tsu_kernels.hip", "tsu_api.cpp", "tsu_run.cpp", "tsu_netcdf.cpp", "tsu_ensemble.cpp", "tsu_datadat.cpp"]
+CLI = os.path.join(HERE, "tsuner")
 
 COMMON = ["-O3", "-fPIC", "-std=c++17", "-ffp-contract=off", "-fno-fast-math", "-fvisibility=hidden",
           "-Wall", "-Wno-unused-function", "-I" + CSRC, "-I" + os.path.join(os.path.dirname(HERE), "include")]
@@ -46,8 +47,16 @@ def build(verbose: bool = False) -> str:
     if r.returncode != 0:
         raise RuntimeError("link failed: %s\n%s\n%s" % (" ".join(cmd), r.stdout, r.stderr))
     os.replace(tmp, LIB)
+    # the `tsuner data.dat [device]` program, linked against the library
+    cmd = ["g++", "-O2", "-std=c++17", "-I" + os.path.join(os.path.dirname(HERE), "include"),
+           os.path.join(CSRC, "tsu_cli.cpp"), "-o", CLI + ".tmp", "-L" + HERE, "-ltsuner",
+           "-Wl,-rpath,$ORIGIN"]
+    r = subprocess.run(cmd, capture_output=True, text=True)
+    if r.returncode != 0:
+        raise RuntimeError("cli build failed: %s\n%s\n%s" % (" ".join(cmd), r.stdout, r.stderr))
+    os.replace(CLI + ".tmp", CLI)
     if verbose:
-        print("built", LIB)
+        print("built", LIB, "and", CLI)
     return LIB
 
 
