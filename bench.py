@@ -89,6 +89,48 @@ def run_variant(sc, precision, steps, warmup, tile_rows):
     return res
 
 
+ENSEMBLE_SIDES = (256, 384, 512, 768, 1024)
+
+
+def run_ensemble(members, steps, seed=2024, chunk=50):
+    """Fine-grid ensemble: `members` Gaussian-source scenarios with sides drawn
+    uniformly from ENSEMBLE_SIDES (dx = 10 m), FP32 state, tile heights
+    autotuned per grid-size class, every member advanced `steps` steps by one
+    batched stage launch per RK stage replayed from HIP graphs.  sims/hr =
+    members * 3600 / wall seconds of the whole ensemble run."""
+    from tsuner_amd import autotune
+    from tsuner_amd import scenarios as S
+    from tsuner_amd.ensemble import Ensemble
+    rng = np.random.default_rng(seed)
+    sides = rng.choice(ENSEMBLE_SIDES, size=members)
+    log("[bench] ensemble: generating %d members" % members)
+    t0 = time.perf_counter()
+    scs = [S.gaussian_source(int(n), 10.0, seed=seed + k, steps=steps) for k, n in enumerate(sides)]
+    cells = int(sum(int(n) * int(n) for n in sides))
+    log("[bench] ensemble: generated in %.1f s, %.1f M cells" % (time.perf_counter() - t0, cells / 1e6))
+    ens = Ensemble(scs, precision="fp32")
+    del scs
+    rows = autotune.tune_ensemble(ens, "fp32", use_cache=False, log=log)
+    ens.set_stop_step(steps)
+    ens.set_graph_chunk(chunk)
+    ens.synchronize()
+    t0 = time.perf_counter()
+    ens.step(steps)
+    wall = time.perf_counter() - t0
+    steps_done = [ens.time(i)[2] for i in range(members)]
+    ens.close()
+    res = {"sims_per_hr": members * 3600.0 / wall, "wall_s": wall, "members": members, "steps": steps,
+           "cells": cells, "cell_updates_per_s": cells * steps / wall,
+           "hbm_gbs_algorithmic": cells * steps * 100.0 / wall / 1e9,
+           "tile_rows": {str(k): v for k, v in rows.items()},
+           "all_members_finished": all(s == steps for s in steps_done),
+           "workload": "%d Gaussian-source members, sides uniform in %s, dx=10 m, fractal shelf bed, sediment on, "
+                       "walls, FP32 state (compensated h/H/hc), %d steps each" % (members, list(ENSEMBLE_SIDES), steps)}
+    log("[bench] ensemble: %.1f s wall, %.0f sims/hr, %.3f G cell-updates/s"
+        % (wall, res["sims_per_hr"], res["cell_updates_per_s"] / 1e9))
+    return res
+
+
 def cpu_baseline(sc, budget_s=12.0):
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
     import pyoracle as O
@@ -124,6 +166,9 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-active", action="store_true",
                     help="skip the active-flow variant (every wet cell moving, sediment suspended)")
+    ap.add_argument("--no-ensemble", action="store_true", help="skip the 256-member fine ensemble")
+    ap.add_argument("--ensemble-members", type=int, default=256)
+    ap.add_argument("--ensemble-steps", type=int, default=3000)
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
         print("bench.py measures one GPU only: this solver path runs on a single MI355X and no multi-GPU "
@@ -153,6 +198,9 @@ def main():
                   "fp32_roofline_frac": None if a32 is None else a32["roofline"]["frac"],
                   "state": "initial state plus u = 1 m/s, v = 0.25 m/s and c = 1e-3 in every wet cell"}
         del sa
+    ensemble = None
+    if not args.no_ensemble:
+        ensemble = run_ensemble(args.ensemble_members, args.ensemble_steps)
     cpu = None
     if not args.no_cpu:
         log("[bench] cpu baseline")
@@ -184,6 +232,7 @@ def main():
                                           "roofline": r32["roofline"]},
         "stage_ms_mean": r64["stage_ms_mean"],
         "active_flow": active,
+        "ensemble_fine": ensemble,
     }
     print(json.dumps(out), flush=True)
 

@@ -149,6 +149,10 @@ TSU_API int tsu_get_time(tsu_state *st, double *t, double *dt, long long *steps)
 TSU_API int tsu_set_time(tsu_state *st, double t, double dt);
 /* Steps enqueued after the simulation time reaches stop_time do nothing. */
 TSU_API int tsu_set_stop_time(tsu_state *st, double stop_time);
+/* on != 0: the step that would cross stop_time is shortened to end on it
+ * exactly (dt = stop_time - t).  Off by default, as in the reference loop
+ * (GPU/main_TSUNER.cu:157-197), whose last step may overshoot. */
+TSU_API int tsu_set_clamp(tsu_state *st, int on);
 /* FP64 sums of h and of (H + hc) over all cells. */
 TSU_API int tsu_mass(tsu_state *st, double out[2]);
 TSU_API int tsu_get_extrema(tsu_state *st, double *hmax, double *bmax, double *eta_max);
@@ -197,6 +201,71 @@ TSU_API int tsu_ensemble_set_graph_chunk(tsu_ensemble *e, int chunk);
 TSU_API int tsu_ensemble_snapshot(tsu_ensemble *e, int restore);
 /* hipEvent-timed wall time of nsteps eager whole-ensemble steps. */
 TSU_API int tsu_ensemble_time_steps(tsu_ensemble *e, int nsteps, double *ms);
+/* Per-member stop times (n doubles); clamp as in tsu_set_clamp. */
+TSU_API int tsu_ensemble_set_stop_time(tsu_ensemble *e, const double *stop_time, int clamp);
+/* Running maxima of h, H and eta of member (schemes with extrema on),
+ * updated after every step as in GPU/Volume_kernel.cu:79-137. */
+TSU_API int tsu_ensemble_get_extrema(tsu_ensemble *e, int member, double *hmax, double *bmax, double *eta_max);
+TSU_API void *tsu_ensemble_stream(tsu_ensemble *e);
+/* Times and step counts of all members (n each; either may be NULL), one
+ * synchronisation. */
+TSU_API int tsu_ensemble_get_times(tsu_ensemble *e, double *t, long long *steps);
+
+/* ---- learned coarse-grid correction (no counterpart in the reference) ------
+ * A 4-layer 3x3 convolutional network (8 -> 32 -> 32 -> 32 -> 4 channels,
+ * ReLU) run on BF16 MFMA with FP32 accumulation.  Activations are NHWC BF16
+ * (uint16 bit patterns), n images of h x w pixels.  Weights: BF16
+ * [9 taps][out][in], tap = 3*(dy+1) + (dx+1); the output layer holds 16 rows
+ * (4 used).  All pointers are device pointers; stream is a hipStream_t (NULL:
+ * the null stream).  Calls only enqueue work. */
+enum {
+    TSU_NN_FWD_IN = 0,     /* x[8] -> relu(conv + b)[32]                       */
+    TSU_NN_FWD_MID = 1,    /* [32] -> relu(conv + b)[32]                        */
+    TSU_NN_FWD_OUT = 2,    /* [32] -> conv + b, FP32 [4]                        */
+    TSU_NN_DGRAD_OUT = 3,  /* dy[8] (4 used) -> dx[32] * (act > 0); w: [9][32][8] */
+    TSU_NN_DGRAD_MID = 4,  /* dy[32] -> dx[32] * (act > 0); w: [9][32][32]      */
+    TSU_NN_WGRAD_IN = 5,   /* dW[9][32][8], db[32] from x[8], dy[32]            */
+    TSU_NN_WGRAD_MID = 6,  /* dW[9][32][32], db[32] from x[32], dy[32]          */
+    TSU_NN_WGRAD_OUT = 7   /* dW[9][mrows][32] rows 0..7, db[8] from x[32], dy[8] */
+};
+/* Convolution kinds 0-4.  act: the layer input's activations (ReLU mask of
+ * the data-gradient kinds), else unused. */
+TSU_API int tsu_nn_conv(int kind, const void *in, const void *w, const float *bias, const void *act, void *out,
+                        int n, int h, int w_, void *stream);
+/* Weight-gradient kinds 5-7 (FP32 dW, db); workspace: FP32 scratch of
+ * tsu_nn_wgrad_workspace(kind) floats. */
+TSU_API long long tsu_nn_wgrad_workspace(int kind);
+TSU_API int tsu_nn_wgrad(int kind, const void *x, const void *dy, float *dw, float *db, int mrows, int n, int h,
+                         int w_, float *workspace, long long workspace_floats, void *stream);
+/* dy[p][0..3] = BF16(scale*(out - target)), dy[p][4..7] = 0 for npix pixels
+ * of 4 FP32 channels; loss_part[b] (nblocks doubles) = partial sums of the
+ * squared error. */
+TSU_API int tsu_nn_mse(const float *out, const float *target, void *dy, double *loss_part, int nblocks,
+                       long long npix, float scale, void *stream);
+/* Adam step on n FP32 parameters; lr_t includes the bias correction. */
+TSU_API int tsu_nn_adam(float *p, const float *g, float *m, float *v, long long n, float lr_t, float beta1,
+                        float beta2, float eps, void *stream);
+/* FP32 [9][m][k] -> BF16 same layout (wb) and, if wt != NULL, the per-tap
+ * transpose wt[9][k][mt] of rows m < mt. */
+TSU_API int tsu_nn_pack(const float *w, void *wb, void *wt, int m, int k, int mt, void *stream);
+/* Network inputs of every member of a two-stage ensemble whose members share
+ * one grid: x[member][y][x][8] BF16 = (h, qx, qy, z = -H, dz/dx, dz/dy, eta if
+ * h > wet_h else 0, wet flag) times scale_in[0..7]. */
+TSU_API int tsu_ensemble_nn_features(tsu_ensemble *e, const float scale_in[8], float wet_h, void *x);
+/* Adds out[member][y][x][0..3] * scale_out to (h, qx, qy, H) of the members
+ * whose step count equals expect_step; h is clipped at 0, a dry cell is only
+ * wetted by more than wet_h, q and hc vanish where h = 0.  resid_max (n
+ * floats on the device, or NULL) is raised to the largest |correction of
+ * h - H| applied to each member (the fallback indicator). */
+TSU_API int tsu_ensemble_nn_apply(tsu_ensemble *e, const float scale_out[4], float wet_h, const float *out,
+                                  long long expect_step, float *resid_max);
+/* coarse member i = 4x4 block means of fine member i (fine grids 4x the
+ * coarse grid in each direction, same precision), then the coarse dt. */
+TSU_API int tsu_ensemble_restrict(tsu_ensemble *fine, tsu_ensemble *coarse);
+/* target[i][y][x][0..3] = (restrict(fine_i) - coarse_i) / scale_out for
+ * (h, qx, qy, H), FP32. */
+TSU_API int tsu_ensemble_nn_target(tsu_ensemble *fine, tsu_ensemble *coarse, const float scale_out[4],
+                                   float *target);
 
 /* Achievable HBM bandwidth: times reps launches of a plain device copy of
  * `bytes` bytes and returns (read + write bytes) / mean launch time in GB/s. */

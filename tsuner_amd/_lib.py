@@ -112,6 +112,26 @@ _PROTOS = {
     "tsu_ensemble_set_graph_chunk": (C.c_int, [C.c_void_p, C.c_int]),
     "tsu_ensemble_snapshot": (C.c_int, [C.c_void_p, C.c_int]),
     "tsu_ensemble_time_steps": (C.c_int, [C.c_void_p, C.c_int, _dp]),
+    "tsu_set_clamp": (C.c_int, [C.c_void_p, C.c_int]),
+    "tsu_ensemble_set_stop_time": (C.c_int, [C.c_void_p, _dp, C.c_int]),
+    "tsu_ensemble_get_extrema": (C.c_int, [C.c_void_p, C.c_int, _dp, _dp, _dp]),
+    "tsu_ensemble_stream": (C.c_void_p, [C.c_void_p]),
+    "tsu_ensemble_get_times": (C.c_int, [C.c_void_p, _dp, C.POINTER(C.c_longlong)]),
+    "tsu_nn_conv": (C.c_int, [C.c_int, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_int, C.c_int,
+                              C.c_int, C.c_void_p]),
+    "tsu_nn_wgrad_workspace": (C.c_longlong, [C.c_int]),
+    "tsu_nn_wgrad": (C.c_int, [C.c_int, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_int, C.c_int, C.c_int,
+                               C.c_int, C.c_void_p, C.c_longlong, C.c_void_p]),
+    "tsu_nn_mse": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_int, C.c_longlong, C.c_float,
+                             C.c_void_p]),
+    "tsu_nn_adam": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_longlong, C.c_float, C.c_float,
+                              C.c_float, C.c_float, C.c_void_p]),
+    "tsu_nn_pack": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_int, C.c_int, C.c_int, C.c_void_p]),
+    "tsu_ensemble_nn_features": (C.c_int, [C.c_void_p, C.POINTER(C.c_float), C.c_float, C.c_void_p]),
+    "tsu_ensemble_nn_apply": (C.c_int, [C.c_void_p, C.POINTER(C.c_float), C.c_float, C.c_void_p, C.c_longlong,
+                                        C.c_void_p]),
+    "tsu_ensemble_restrict": (C.c_int, [C.c_void_p, C.c_void_p]),
+    "tsu_ensemble_nn_target": (C.c_int, [C.c_void_p, C.c_void_p, C.POINTER(C.c_float), C.c_void_p]),
     "tsu_bandwidth_probe": (C.c_int, [C.c_int, C.c_longlong, C.c_int, _dp]),
     "tsu_parse_data_dat": (C.c_int, [C.c_char_p, C.POINTER(SimParams), C.c_char_p, C.c_int, C.c_char_p, C.c_int,
                                      _dp]),

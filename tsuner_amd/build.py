@@ -19,7 +19,8 @@ BUILD = os.path.join(HERE, "_build")
 LIB = os.path.join(HERE, "libtsuner.so")
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 
-SOURCES = ["tsu_kernels.hip", "tsu_api.cpp", "tsu_run.cpp", "tsu_netcdf.cpp", "tsu_ensemble.cpp", "tsu_datadat.cpp"]
+SOURCES = ["tsu_kernels.hip", "tsu_api.cpp", "tsu_run.cpp", "tsu_netcdf.cpp", "tsu_ensemble.cpp", "tsu_datadat.cpp",
+           "tsu_nn.hip", "tsu_nn_api.cpp"]
 CLI = os.path.join(HERE, "tsuner")
 
 COMMON = ["-O3", "-fPIC", "-std=c++17", "-ffp-contract=off", "-fno-fast-math", "-fvisibility=hidden",

@@ -1,0 +1,410 @@
+"""Learned coarse-grid correction for ensembles: the solver on a 4x coarser
+grid plus a convolutional network that, every WINDOW coarse steps, adds the
+difference between the restricted fine solution and the coarse one.
+
+Pipeline (all on the GPU; torch only allocates buffers):
+  1. data: fine members (1024^2, dx 10 m) and their 4x4 restrictions (256^2,
+     dx 40 m) advance window by window.  At each window start the coarse
+     state is reset to restrict(fine); after WINDOW coarse steps the fine
+     members advance to exactly the coarse members' times (clamped stop
+     times); the pair is (network inputs of the coarse state,
+     (restrict(fine) - coarse) / scale_out).
+  2. training: Adam on the library's forward / backward kernels.
+  3. deployment: a coarse ensemble steps WINDOW steps, the network's output
+     is added to every member that completed the window, repeat.  Members
+     whose correction exceeds a residual threshold are flagged and re-run
+     with the fine solver (fallback).
+
+Scenarios: tsuner_amd.scenarios.gaussian_source (fractal shelf bed, Gaussian
+eta source, walls, sediment on).
+"""
+from __future__ import annotations
+
+import ctypes as C
+import json
+import os
+import time
+from dataclasses import dataclass, field
+from typing import List, Sequence
+
+import numpy as np
+
+from . import nn
+from . import scenarios as S
+
+FINE_N, FACTOR, FINE_DX = 1024, 4, 10.0
+COARSE_N, COARSE_DX = FINE_N // FACTOR, FINE_DX * FACTOR
+WINDOW = 10
+WET_H = 1e-3
+# inputs: h, qx, qy, z, dz/dx, dz/dy, eta (wet), wet flag
+SCALE_IN = [1 / 50.0, 1 / 5.0, 1 / 5.0, 1 / 50.0, 20.0, 20.0, 2.0, 1.0]
+HERE = os.path.dirname(os.path.abspath(__file__))
+WEIGHTS = os.path.join(HERE, "correction_weights.npz")
+
+
+def log(msg):
+    print(msg, flush=True)
+
+
+def coarse_scenario(fine: S.Scenario) -> S.Scenario:
+    """4x4 conservative restriction of a fine scenario (block means of every
+    state plane; a coarse cell is erodible only if all its fine cells are)."""
+    n = fine.npx // FACTOR
+    p = dict(fine.params)
+    p.update(npx=n, npy=n, nvol=n * n, dx=fine.params["dx"] * FACTOR, dy=fine.params["dy"] * FACTOR)
+    V = fine.V.reshape(5, n, FACTOR, n, FACTOR).mean(axis=(2, 4))
+    fix = fine.fixH.reshape(n, FACTOR, n, FACTOR).min(axis=(1, 3)).astype(np.int32)
+    return S.Scenario(fine.name + "_coarse", p, dict(fine.scheme), np.ascontiguousarray(V), fix, steps=fine.steps)
+
+
+def fine_scenario(seed: int, extrema: int = 0) -> S.Scenario:
+    return S.gaussian_source(FINE_N, FINE_DX, seed=seed, extrema=extrema)
+
+
+class _Stream:
+    """Makes torch's current stream the ensemble's stream, so library calls
+    made through torch-allocated buffers run in order with the solver."""
+
+    def __init__(self, ens):
+        import torch
+        self.ctx = torch.cuda.stream(torch.cuda.ExternalStream(ens.stream))
+
+    def __enter__(self):
+        self.ctx.__enter__()
+
+    def __exit__(self, *a):
+        self.ctx.__exit__(*a)
+
+
+def advance_fine_to(fine, times: np.ndarray, chunk: int = 16, max_steps: int = 100000):
+    """Steps the fine ensemble until every member's time equals its entry of
+    `times` (clamped stop times)."""
+    fine.set_stop_time(times, clamp=True)
+    done = 0
+    while done < max_steps:
+        fine.step(chunk)
+        done += chunk
+        if np.all(fine.times() >= times):
+            return done
+    raise RuntimeError("fine members did not reach the window end")
+
+
+@dataclass
+class Dataset:
+    x: object            # torch BF16 [N][n][n][8]
+    y: object            # torch FP32 [N][n][n][4]
+    scale_out: List[float]
+    count: int
+    seeds: List[int] = field(default_factory=list)
+
+
+def generate(seeds: Sequence[int], windows: int, scale_out: Sequence[float], precision: str = "fp32",
+             capacity: int = None, seed_perm: int = 0) -> Dataset:
+    """Training pairs from fine/coarse runs of `seeds`, `windows` windows each.
+    Pairs are written in a random order (contiguous batches are then random)."""
+    import torch
+    from .ensemble import Ensemble
+    fine_sc = [fine_scenario(s) for s in seeds]
+    coarse_sc = [coarse_scenario(f) for f in fine_sc]
+    m = len(seeds)
+    total = m * windows if capacity is None else capacity
+    npx = COARSE_N
+    X = torch.zeros((total, npx, npx, 8), dtype=torch.bfloat16, device="cuda")
+    Y = torch.zeros((total, npx, npx, 4), dtype=torch.float32, device="cuda")
+    perm = np.random.default_rng(seed_perm).permutation(total)
+    fine = Ensemble(fine_sc, precision=precision)
+    coarse = Ensemble(coarse_sc, precision=precision)
+    del fine_sc, coarse_sc
+    xb = torch.zeros((m, npx, npx, 8), dtype=torch.bfloat16, device="cuda")
+    yb = torch.zeros((m, npx, npx, 4), dtype=torch.float32, device="cuda")
+    torch.cuda.synchronize()
+    k = 0
+    t0 = time.perf_counter()
+    for w in range(windows):
+        if w > 0:
+            coarse.restrict_from(fine)
+        coarse.step(WINDOW)
+        tc = coarse.times()
+        advance_fine_to(fine, tc)
+        coarse.nn_features(SCALE_IN, WET_H, xb.data_ptr())
+        coarse.nn_target(fine, scale_out, yb.data_ptr())
+        coarse.synchronize()
+        for i in range(m):
+            if k < total:
+                X[perm[k]].copy_(xb[i])
+                Y[perm[k]].copy_(yb[i])
+            k += 1
+        torch.cuda.synchronize()
+        if w % 10 == 0:
+            log("[data] window %d/%d t=%.1f s (%.1f s)" % (w + 1, windows, float(tc.mean()), time.perf_counter() - t0))
+    fine.close()
+    coarse.close()
+    return Dataset(X, Y, list(scale_out), min(k, total), list(seeds))
+
+
+def pilot_scales(seeds: Sequence[int], windows: int) -> List[float]:
+    """RMS of the raw corrections (h, qx, qy, H) over a short pilot run."""
+    ds = generate(seeds, windows, [1.0, 1.0, 1.0, 1.0])
+    y = ds.y[:ds.count].cpu().numpy().reshape(-1, 4).astype(np.float64)
+    rms = np.sqrt(np.mean(y * y, axis=0))
+    return [float(max(r, 1e-6)) for r in rms]
+
+
+def train(ds: Dataset, epochs: int, batch: int = 8, lr: float = 2e-3, seed: int = 0, net: "nn.Net" = None,
+          log_every: int = 1):
+    """Adam over contiguous batches of the (pre-shuffled) dataset; the batch
+    order is reshuffled every epoch.  Returns the network and loss history."""
+    n = ds.count
+    nb = n // batch
+    h = ds.x.shape[1]
+    if net is None:
+        net = nn.Net(nn.init_params(seed), max_batch=batch, h=h, w=h)
+    rng = np.random.default_rng(seed)
+    hist = []
+    base = ds.x.data_ptr(), ds.y.data_ptr()
+    xstride = h * h * 8 * 2
+    ystride = h * h * 4 * 4
+    t0 = time.perf_counter()
+    for ep in range(epochs):
+        lr_ep = lr * 0.5 * (1.0 + np.cos(np.pi * ep / max(1, epochs)))
+        for b in rng.permutation(nb):
+            xa = base[0] + int(b) * batch * xstride
+            ya = base[1] + int(b) * batch * ystride
+            net.forward(xa, batch)
+            net.backward(xa, ya, batch)
+            net.adam(lr_ep)
+        # loss of the epoch's last batch (one synchronisation per epoch)
+        loss = net.loss(batch)
+        hist.append(loss)
+        if ep % log_every == 0:
+            log("[train] epoch %d/%d lr %.2e loss %.4f (%.1f s)" % (ep + 1, epochs, lr_ep, loss,
+                                                                   time.perf_counter() - t0))
+    return net, hist
+
+
+@dataclass
+class RunResult:
+    wall_s: float
+    t_end: np.ndarray                 # per member
+    eta_max: List[np.ndarray]         # per member, NaN where never wet
+    hmax: List[np.ndarray]
+    H0: List[np.ndarray]              # initial depth below datum
+    resid: np.ndarray = None          # per member max |correction of eta| (coarse + network)
+    steps: np.ndarray = None
+
+
+def _extrema(ens, i):
+    hmax, _, emax = ens.extrema(i)
+    emax = np.where(emax > -1e29, emax, np.nan)
+    hmax = np.where(hmax > -1e29, hmax, 0.0)
+    return emax, hmax
+
+
+def run_fine(scs: Sequence[S.Scenario], steps: int, precision: str = "fp32", chunk: int = 50) -> RunResult:
+    """Fine ensemble, `steps` steps per member, extrema on; wall time of the
+    stepping only (states already resident)."""
+    from .ensemble import Ensemble
+    ens = Ensemble(scs, precision=precision)
+    ens.set_stop_step(steps)
+    ens.set_graph_chunk(chunk)
+    ens.synchronize()
+    t0 = time.perf_counter()
+    ens.step(steps)
+    wall = time.perf_counter() - t0
+    res = RunResult(wall, ens.times(), [], [], [sc.V[3].copy() for sc in scs], steps=ens.steps_done())
+    for i in range(len(scs)):
+        e, h = _extrema(ens, i)
+        res.eta_max.append(e)
+        res.hmax.append(h)
+    ens.close()
+    return res
+
+
+def run_coarse(scs: Sequence[S.Scenario], t_end: np.ndarray, net: "nn.Net" = None, scale_out=None,
+               precision: str = "fp32", max_windows: int = 100000) -> RunResult:
+    """Coarse ensemble to the per-member end times (clamped), with the
+    network's correction after every WINDOW steps when `net` is given."""
+    import torch
+    from .ensemble import Ensemble
+    n = len(scs)
+    ens = Ensemble(scs, precision=precision)
+    ens.set_stop_time(np.asarray(t_end, np.float64), clamp=True)
+    npx = scs[0].npx
+    x = out = resid = None
+    if net is not None:
+        x = torch.zeros((n, npx, npx, 8), dtype=torch.bfloat16, device="cuda")
+        out = torch.zeros((n, npx, npx, 4), dtype=torch.float32, device="cuda")
+        resid = torch.zeros(n, dtype=torch.float32, device="cuda")
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    w = 0
+    # Members advance in lockstep (WINDOW steps per window) until their own
+    # clamped end time; a member is corrected only if it completed the window.
+    with _Stream(ens):
+        while w < max_windows:
+            w += 1
+            ens.step(WINDOW, sync=False)
+            if net is not None:
+                ens.nn_features(SCALE_IN, WET_H, x.data_ptr())
+                for b0 in range(0, n, net.max_batch):
+                    nb = min(net.max_batch, n - b0)
+                    net.forward(x.data_ptr() + b0 * npx * npx * 16, nb, out=out.data_ptr() + b0 * npx * npx * 16)
+                ens.nn_apply(scale_out, WET_H, out.data_ptr(), WINDOW * w, resid.data_ptr())
+            if np.all(ens.times() >= t_end):
+                break
+    wall = time.perf_counter() - t0
+    res = RunResult(wall, ens.times(), [], [], [sc.V[3].copy() for sc in scs], steps=ens.steps_done())
+    if resid is not None:
+        res.resid = resid.cpu().numpy()
+    for i in range(n):
+        e, h = _extrema(ens, i)
+        res.eta_max.append(e)
+        res.hmax.append(h)
+    ens.close()
+    return res
+
+
+def _block_mean_nan(a: np.ndarray) -> np.ndarray:
+    n = a.shape[0] // FACTOR
+    b = a.reshape(n, FACTOR, n, FACTOR)
+    with np.errstate(invalid="ignore"):
+        return np.nanmean(b, axis=(1, 3)) if np.isfinite(b).any() else np.full((n, n), np.nan)
+
+
+def runup(H0: np.ndarray, hmax: np.ndarray) -> float:
+    """Largest ground elevation (z = -H > 0) reached by water (h > WET_H)."""
+    land = H0 < 0
+    wet = hmax > WET_H
+    sel = land & wet
+    return float((-H0[sel]).max()) if sel.any() else 0.0
+
+
+def metrics(fine: RunResult, coarse: RunResult, i: int, nearshore_depth: float = 10.0) -> dict:
+    """Errors of coarse member i against fine member i: peak eta_max over the
+    nearshore zone (coarse cells with initial depth < nearshore_depth,
+    including land), mean |eta_max error| there, and runup."""
+    import warnings
+    with warnings.catch_warnings():
+        warnings.simplefilter("ignore", RuntimeWarning)
+        ef = _block_mean_nan(fine.eta_max[i])
+    ec = coarse.eta_max[i]
+    zone = coarse.H0[i] < nearshore_depth
+    both = zone & np.isfinite(ef) & np.isfinite(ec)
+    peak_f = float(np.nanmax(np.where(zone, ef, np.nan))) if np.isfinite(ef[zone]).any() else float("nan")
+    peak_c = float(np.nanmax(np.where(zone, ec, np.nan))) if np.isfinite(ec[zone]).any() else float("nan")
+    rf = runup(fine.H0[i], fine.hmax[i])
+    rc = runup(coarse.H0[i], coarse.hmax[i])
+    return {"peak_eta_fine": peak_f, "peak_eta_coarse": peak_c, "peak_eta_err": abs(peak_c - peak_f),
+            "mean_abs_eta_err": float(np.mean(np.abs(ec[both] - ef[both]))) if both.any() else float("nan"),
+            "runup_fine": rf, "runup_coarse": rc,
+            "runup_rel_err": abs(rc - rf) / rf if rf > 0 else (0.0 if rc == 0 else float("inf"))}
+
+
+def save(path: str, net: "nn.Net", scale_out, info: dict):
+    np.savez(path, params=net.params(), scale_out=np.asarray(scale_out, np.float64),
+             scale_in=np.asarray(SCALE_IN, np.float64), info=json.dumps(info))
+
+
+def load(path: str = WEIGHTS):
+    with np.load(path, allow_pickle=False) as f:
+        return f["params"].astype(np.float32), [float(v) for v in f["scale_out"]], json.loads(str(f["info"]))
+
+
+def evaluate(net, scale_out, seeds: Sequence[int], fine_steps: int = 3000, resid_tau: float = 0.5,
+             max_batch: int = 32) -> dict:
+    """Held-out evaluation: fine members (fine_steps steps) against the coarse
+    solver alone and coarse + network, with the residual fallback."""
+    t0 = time.perf_counter()
+    fine_sc = [fine_scenario(s, extrema=1) for s in seeds]
+    coarse_sc = [coarse_scenario(f) for f in fine_sc]
+    log("[eval] %d scenarios generated (%.1f s)" % (len(seeds), time.perf_counter() - t0))
+    fine = run_fine(fine_sc, fine_steps)
+    del fine_sc
+    log("[eval] fine: %.2f s wall, %.0f sims/hr" % (fine.wall_s, len(seeds) * 3600 / fine.wall_s))
+    base = run_coarse(coarse_sc, fine.t_end)
+    log("[eval] coarse alone: %.2f s wall" % base.wall_s)
+    corr = run_coarse(coarse_sc, fine.t_end, net=net, scale_out=scale_out)
+    log("[eval] coarse + network: %.2f s wall" % corr.wall_s)
+    mb = [metrics(fine, base, i) for i in range(len(seeds))]
+    mc = [metrics(fine, corr, i) for i in range(len(seeds))]
+    flagged = corr.resid > resid_tau
+    n = len(seeds)
+    fine_per = fine.wall_s / n
+    wall_fallback = corr.wall_s + fine_per * int(flagged.sum())
+
+    def summary(ms, use_fine=None):
+        pe = np.array([0.0 if (use_fine is not None and use_fine[i]) else m["peak_eta_err"] for i, m in enumerate(ms)])
+        me = np.array([0.0 if (use_fine is not None and use_fine[i]) else m["mean_abs_eta_err"] for i, m in enumerate(ms)])
+        ru = np.array([0.0 if (use_fine is not None and use_fine[i]) else m["runup_rel_err"] for i, m in enumerate(ms)])
+        ok = np.isfinite(pe)
+        okr = np.isfinite(ru)
+        return {"peak_eta_err_mean_m": float(np.mean(pe[ok])), "peak_eta_err_p95_m": float(np.percentile(pe[ok], 95)),
+                "peak_eta_err_max_m": float(np.max(pe[ok])),
+                "nearshore_mean_abs_eta_err_m": float(np.nanmean(me)),
+                "runup_rel_err_mean": float(np.mean(ru[okr])), "runup_rel_err_p95": float(np.percentile(ru[okr], 95)),
+                "frac_peak_eta_within_5cm": float(np.mean(pe[ok] <= 0.05)),
+                "frac_runup_within_10pct": float(np.mean(ru[okr] <= 0.10))}
+
+    out = {
+        "scenarios": n, "fine_steps": fine_steps,
+        "fine": {"wall_s": fine.wall_s, "sims_per_hr": n * 3600 / fine.wall_s},
+        "coarse_alone": dict(summary(mb), wall_s=base.wall_s, sims_per_hr=n * 3600 / base.wall_s),
+        "coarse_nn": dict(summary(mc), wall_s=corr.wall_s, sims_per_hr=n * 3600 / corr.wall_s,
+                          speedup_vs_fine=fine.wall_s / corr.wall_s),
+        "coarse_nn_with_fallback": dict(summary(mc, flagged), resid_tau_m=resid_tau,
+                                        fallback_frac=float(flagged.mean()), wall_s=wall_fallback,
+                                        sims_per_hr=n * 3600 / wall_fallback,
+                                        speedup_vs_fine=fine.wall_s / wall_fallback),
+        "resid_max_m": {"p50": float(np.median(corr.resid)), "p95": float(np.percentile(corr.resid, 95)),
+                        "max": float(corr.resid.max())},
+    }
+    return out
+
+
+def main(argv=None):
+    import argparse
+    ap = argparse.ArgumentParser(description="learned coarse-grid correction: train / evaluate")
+    ap.add_argument("cmd", choices=["train", "evaluate", "all"])
+    ap.add_argument("--seeds", type=int, default=28, help="training scenarios")
+    ap.add_argument("--windows", type=int, default=75, help="windows per training scenario")
+    ap.add_argument("--epochs", type=int, default=30)
+    ap.add_argument("--batch", type=int, default=8)
+    ap.add_argument("--lr", type=float, default=2e-3)
+    ap.add_argument("--heldout", type=int, default=200)
+    ap.add_argument("--fine-steps", type=int, default=3000)
+    ap.add_argument("--resid-tau", type=float, default=0.5)
+    ap.add_argument("--weights", default=WEIGHTS)
+    ap.add_argument("--json", default=None)
+    a = ap.parse_args(argv)
+    import torch
+    torch.cuda.init()
+    result = {}
+    if a.cmd in ("train", "all"):
+        t0 = time.perf_counter()
+        scale_out = pilot_scales([900001, 900002], 20)
+        log("[train] output scales %s" % scale_out)
+        ds = generate(list(range(1, a.seeds + 1)), a.windows, scale_out)
+        t_data = time.perf_counter() - t0
+        log("[train] %d pairs in %.1f s" % (ds.count, t_data))
+        t1 = time.perf_counter()
+        net, hist = train(ds, a.epochs, batch=a.batch, lr=a.lr)
+        t_train = time.perf_counter() - t1
+        info = {"pairs": ds.count, "train_seeds": ds.seeds, "windows": a.windows, "epochs": a.epochs,
+                "batch": a.batch, "lr": a.lr, "loss": hist, "data_s": t_data, "train_s": t_train}
+        save(a.weights, net, scale_out, info)
+        result["train"] = {k: v for k, v in info.items() if k != "train_seeds"}
+        del ds
+    if a.cmd in ("evaluate", "all"):
+        params, scale_out, _ = load(a.weights)
+        net = nn.Net(params, max_batch=32, h=COARSE_N, w=COARSE_N)
+        seeds = list(range(100001, 100001 + a.heldout))
+        result["evaluate"] = evaluate(net, scale_out, seeds, fine_steps=a.fine_steps, resid_tau=a.resid_tau)
+    js = json.dumps(result, indent=1)
+    print(js, flush=True)
+    if a.json:
+        with open(a.json, "w") as f:
+            f.write(js)
+
+
+if __name__ == "__main__":
+    main()

@@ -613,6 +613,17 @@ TSU_API int tsu_set_stop_time(tsu_state* st, double stop_time)
     return TSU_OK;
 }
 
+TSU_API int tsu_set_clamp(tsu_state* st, int on)
+{
+    if (!st) return fail(TSU_ERR_ARG, "null state");
+    DeviceGuard g(st->device);
+    const long long v = on ? 1 : 0;
+    HIPCHK(hipMemcpyAsync((char*)st->ctrl + offsetof(Ctrl, clamp), &v, sizeof(long long), hipMemcpyHostToDevice,
+                          st->stream));
+    HIPCHK(hipStreamSynchronize(st->stream));
+    return TSU_OK;
+}
+
 TSU_API int tsu_mass(tsu_state* st, double out[2])
 {
     if (!st || !out) return fail(TSU_ERR_ARG, "null argument");

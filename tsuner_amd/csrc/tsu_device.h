@@ -81,8 +81,17 @@ struct Ctrl {
     unsigned long long dt_bits[2];
     long long step;
     long long stop_step;   // steps enqueued once step >= stop_step do nothing
-    long long pad[2];
+    long long clamp;       // 1: the step crossing stop_time ends on it (dt = stop_time - t)
+    long long pad;
 };
+
+// dt of the step starting at t: the finalized minimum, shortened to land on
+// stop_time when clamping is on.  Stage and advance kernels use the same rule.
+__device__ __forceinline__ double clamp_dt(const Ctrl* C, double t, double dt)
+{
+    if (C->clamp && !(dt < C->stop_time - t)) return C->stop_time - t;
+    return dt;
+}
 
 // ---- primitive operations per compute type --------------------------------
 

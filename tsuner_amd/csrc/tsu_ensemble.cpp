@@ -17,6 +17,7 @@
 
 #include "tsu_host.h"
 #include "tsu_kernels.h"
+#include "tsu_nn.h"
 
 using tsu::Ctrl;
 using tsu::DevParams;
@@ -44,6 +45,7 @@ struct tsu_ensemble {
     std::vector<DevParams> P;
     std::vector<size_t> off;        // member offset in the state pools (bytes)
     std::vector<size_t> foff;       // member offset in the mask pool
+    std::vector<size_t> xoff;       // member offset in the extrema pool (doubles), if extrema on
     std::vector<int> rows;          // tile rows per member
     std::vector<bool> uploaded;
     size_t esize = 8, planes = 5, pool_bytes = 0, fix_bytes = 0;
@@ -51,6 +53,8 @@ struct tsu_ensemble {
     char* B = nullptr;
     char* snapA = nullptr;          // snapshot for tuning
     uint8_t* fix = nullptr;
+    double* ext = nullptr;          // hmax, max H, eta_max planes per member with extrema on
+    size_t ext_doubles = 0;
     Ctrl* ctrl = nullptr;
     Ctrl* snapC = nullptr;
     DevParams* dP = nullptr;
@@ -98,6 +102,7 @@ void destroy(tsu_ensemble* e)
     (void)hipFree(e->B);
     (void)hipFree(e->snapA);
     (void)hipFree(e->fix);
+    (void)hipFree(e->ext);
     (void)hipFree(e->ctrl);
     (void)hipFree(e->snapC);
     (void)hipFree(e->dP);
@@ -126,7 +131,7 @@ int build_launch_tables(tsu_ensemble* e)
             a.Hin = Ai + 3 * plane;
             a.fix = e->fix + e->foff[i];
             a.ctrl = e->ctrl + i;
-            a.ext = nullptr;
+            a.ext = e->s[i].extrema ? e->ext + e->xoff[i] : nullptr;
             a.tile_rows = e->rows[i];
             a.nstrips = nstrips(e, i);
             a.ntiles = a.nstrips * nchunks(e, i);
@@ -222,6 +227,7 @@ TSU_API int tsu_ensemble_create(tsu_ensemble** out, int n, const tsu_sim_params*
     e->P.resize(n);
     e->off.resize(n);
     e->foff.resize(n);
+    e->xoff.assign(n, 0);
     e->rows.assign(n, 16);
     e->uploaded.assign(n, false);
     long long maxnv = 0;
@@ -234,6 +240,10 @@ TSU_API int tsu_ensemble_create(tsu_ensemble** out, int n, const tsu_sim_params*
         e->fix_bytes += align256((size_t)e->P[i].nvol);
         if (e->P[i].nvol > maxnv) maxnv = e->P[i].nvol;
         e->sed = e->sed || scheme[i].bedload || scheme[i].suspension;
+        if (scheme[i].extrema) {
+            e->xoff[i] = e->ext_doubles;
+            e->ext_doubles += 3 * (size_t)((e->P[i].nvol + 31) / 32 * 32);
+        }
     }
     e->mass_nb = tsu::mass_blocks(maxnv);
     Guard g(device);
@@ -241,6 +251,7 @@ TSU_API int tsu_ensemble_create(tsu_ensemble** out, int n, const tsu_sim_params*
     if (err == hipSuccess) err = hipMalloc((void**)&e->A, e->pool_bytes);
     if (err == hipSuccess) err = hipMalloc((void**)&e->B, e->pool_bytes);
     if (err == hipSuccess) err = hipMalloc((void**)&e->fix, e->fix_bytes);
+    if (err == hipSuccess && e->ext_doubles) err = hipMalloc((void**)&e->ext, e->ext_doubles * sizeof(double));
     if (err == hipSuccess) err = hipMalloc((void**)&e->ctrl, n * sizeof(Ctrl));
     if (err == hipSuccess) err = hipMalloc((void**)&e->dP, n * sizeof(DevParams));
     if (err == hipSuccess) err = hipMalloc((void**)&e->dargs, 2 * (size_t)n * sizeof(tsu::StageArgs));
@@ -311,6 +322,7 @@ TSU_API int tsu_ensemble_upload(tsu_ensemble* e, int i, const double* V, const i
     }
     ECHK(hipMemcpyAsync(e->B + e->off[i], e->A + e->off[i], bytes, hipMemcpyDeviceToDevice, e->stream));
     ECHK(hipMemcpyAsync(e->fix + e->foff[i], fx.data(), (size_t)nv, hipMemcpyHostToDevice, e->stream));
+    if (e->s[i].extrema) ECHK(tsu::launch_fill(e->ext + e->xoff[i], 3 * nv, -1e30, e->stream));
     Ctrl c;
     std::memset(&c, 0, sizeof(c));
     c.stop_time = INFINITY;
@@ -512,6 +524,151 @@ TSU_API int tsu_ensemble_time_steps(tsu_ensemble* e, int nsteps, double* ms)
     if (rc) return rc;
     if (er != hipSuccess) return tsu::set_error(TSU_ERR_HIP, std::string("timing: ") + hipGetErrorString(er));
     *ms = f;
+    return TSU_OK;
+}
+
+TSU_API int tsu_ensemble_set_stop_time(tsu_ensemble* e, const double* stop_time, int clamp)
+{
+    if (!e || !stop_time) return tsu::set_error(TSU_ERR_ARG, "null argument");
+    Guard g(e->device);
+    const long long cl = clamp ? 1 : 0;
+    for (int i = 0; i < e->n; i++) {
+        ECHK(hipMemcpyAsync((char*)(e->ctrl + i) + offsetof(Ctrl, stop_time), stop_time + i, sizeof(double),
+                            hipMemcpyHostToDevice, e->stream));
+        ECHK(hipMemcpyAsync((char*)(e->ctrl + i) + offsetof(Ctrl, clamp), &cl, sizeof(long long),
+                            hipMemcpyHostToDevice, e->stream));
+    }
+    ECHK(hipStreamSynchronize(e->stream));
+    return TSU_OK;
+}
+
+TSU_API int tsu_ensemble_get_extrema(tsu_ensemble* e, int i, double* hmax, double* bmax, double* eta_max)
+{
+    int rc = member_ok(e, i);
+    if (rc) return rc;
+    if (!e->s[i].extrema) return tsu::set_error(TSU_ERR_ARG, "extrema are not enabled for this member");
+    Guard g(e->device);
+    const size_t nv = (size_t)e->P[i].nvol, b = nv * sizeof(double);
+    const double* x = e->ext + e->xoff[i];
+    if (hmax) ECHK(hipMemcpyAsync(hmax, x, b, hipMemcpyDeviceToHost, e->stream));
+    if (bmax) ECHK(hipMemcpyAsync(bmax, x + nv, b, hipMemcpyDeviceToHost, e->stream));
+    if (eta_max) ECHK(hipMemcpyAsync(eta_max, x + 2 * nv, b, hipMemcpyDeviceToHost, e->stream));
+    ECHK(hipStreamSynchronize(e->stream));
+    return TSU_OK;
+}
+
+TSU_API void* tsu_ensemble_stream(tsu_ensemble* e) { return e ? (void*)e->stream : nullptr; }
+
+TSU_API int tsu_ensemble_get_times(tsu_ensemble* e, double* t, long long* steps)
+{
+    if (!e || (!t && !steps)) return tsu::set_error(TSU_ERR_ARG, "bad argument");
+    Guard g(e->device);
+    std::vector<Ctrl> c((size_t)e->n);
+    ECHK(hipMemcpyAsync(c.data(), e->ctrl, e->n * sizeof(Ctrl), hipMemcpyDeviceToHost, e->stream));
+    ECHK(hipStreamSynchronize(e->stream));
+    for (int i = 0; i < e->n; i++) {
+        if (t) t[i] = c[(size_t)i].t;
+        if (steps) steps[i] = c[(size_t)i].step;
+    }
+    return TSU_OK;
+}
+
+}  // extern "C"
+
+// ---- hooks of the learned coarse-grid correction ---------------------------
+
+namespace {
+
+// Members of one grid and a two-stage scheme: the stage-2 arguments' .out is
+// the current state.
+int nn_uniform(const tsu_ensemble* e, const char* what)
+{
+    if (!e) return tsu::set_error(TSU_ERR_ARG, std::string(what) + ": null ensemble");
+    if (e->s[0].order != 2) return tsu::set_error(TSU_ERR_UNSUPPORTED, std::string(what) + ": needs the two-stage scheme");
+    for (int i = 1; i < e->n; i++)
+        if (e->p[i].npx != e->p[0].npx || e->p[i].npy != e->p[0].npy)
+            return tsu::set_error(TSU_ERR_UNSUPPORTED, std::string(what) + ": members must share one grid");
+    for (int i = 0; i < e->n; i++)
+        if (!e->uploaded[i]) return tsu::set_error(TSU_ERR_ARG, std::string(what) + ": member not uploaded");
+    return TSU_OK;
+}
+
+tsu::NnScales make_scales(const float* in8, const float* out4, float wet_h)
+{
+    tsu::NnScales s{};
+    for (int j = 0; j < 8; j++) s.in[j] = in8 ? in8[j] : 0.f;
+    for (int j = 0; j < 4; j++) s.out[j] = out4 ? out4[j] : 1.f;
+    s.wet_h = wet_h;
+    return s;
+}
+
+int nn_pair(const tsu_ensemble* f, const tsu_ensemble* c, const char* what)
+{
+    int rc = nn_uniform(f, what);
+    if (rc) return rc;
+    if ((rc = nn_uniform(c, what))) return rc;
+    if (f->n != c->n || f->precision != c->precision)
+        return tsu::set_error(TSU_ERR_ARG, std::string(what) + ": ensembles differ in size or precision");
+    if (f->p[0].npx != 4 * c->p[0].npx || f->p[0].npy != 4 * c->p[0].npy)
+        return tsu::set_error(TSU_ERR_ARG, std::string(what) + ": fine grid must be 4x the coarse grid");
+    if (f->device != c->device) return tsu::set_error(TSU_ERR_ARG, std::string(what) + ": ensembles on different devices");
+    return TSU_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+TSU_API int tsu_ensemble_nn_features(tsu_ensemble* e, const float scale_in[8], float wet_h, void* x)
+{
+    int rc = nn_uniform(e, "tsu_ensemble_nn_features");
+    if (rc) return rc;
+    if (!scale_in || !x) return tsu::set_error(TSU_ERR_ARG, "null argument");
+    Guard g(e->device);
+    ECHK(tsu::nn_launch_features(e->dargs + e->n, e->n, e->p[0].npx, e->p[0].npy, e->precision,
+                                 make_scales(scale_in, nullptr, wet_h), (uint16_t*)x, e->stream));
+    return TSU_OK;
+}
+
+TSU_API int tsu_ensemble_nn_apply(tsu_ensemble* e, const float scale_out[4], float wet_h, const float* out,
+                                  long long expect_step, float* resid_max)
+{
+    int rc = nn_uniform(e, "tsu_ensemble_nn_apply");
+    if (rc) return rc;
+    if (!scale_out || !out) return tsu::set_error(TSU_ERR_ARG, "null argument");
+    Guard g(e->device);
+    ECHK(tsu::nn_launch_apply(e->dargs + e->n, e->n, e->p[0].npx, e->p[0].npy, e->precision,
+                              make_scales(nullptr, scale_out, wet_h), out, expect_step, resid_max, e->stream));
+    return TSU_OK;
+}
+
+TSU_API int tsu_ensemble_restrict(tsu_ensemble* fine, tsu_ensemble* coarse)
+{
+    int rc = nn_pair(fine, coarse, "tsu_ensemble_restrict");
+    if (rc) return rc;
+    Guard g(coarse->device);
+    ECHK(hipStreamSynchronize(fine->stream));
+    ECHK(tsu::nn_launch_restrict(fine->dargs + fine->n, coarse->dargs + coarse->n, coarse->n, coarse->p[0].npx,
+                                 coarse->p[0].npy, coarse->precision, coarse->stream));
+    // keep the stage-1 buffer equal to the state, as after an upload
+    for (int i = 0; i < coarse->n; i++)
+        ECHK(hipMemcpyAsync(coarse->B + coarse->off[i], coarse->A + coarse->off[i],
+                            coarse->planes * (size_t)coarse->P[i].nvol * coarse->esize, hipMemcpyDeviceToDevice,
+                            coarse->stream));
+    ECHK(hipStreamSynchronize(coarse->stream));
+    return tsu_ensemble_init_dt(coarse);
+}
+
+TSU_API int tsu_ensemble_nn_target(tsu_ensemble* fine, tsu_ensemble* coarse, const float scale_out[4], float* target)
+{
+    int rc = nn_pair(fine, coarse, "tsu_ensemble_nn_target");
+    if (rc) return rc;
+    if (!scale_out || !target) return tsu::set_error(TSU_ERR_ARG, "null argument");
+    Guard g(coarse->device);
+    ECHK(hipStreamSynchronize(fine->stream));
+    ECHK(tsu::nn_launch_target(fine->dargs + fine->n, coarse->dargs + coarse->n, coarse->n, coarse->p[0].npx,
+                               coarse->p[0].npy, coarse->precision, make_scales(nullptr, scale_out, 0.f), target,
+                               coarse->stream));
     return TSU_OK;
 }
 

@@ -229,7 +229,7 @@ __device__ __forceinline__ void stage_body(const StageArgs& a, const int tile)
     const double ct = C->t;
     if (!(ct < C->stop_time) || C->step >= C->stop_step) return;
     const long long step = C->step;
-    const double dtd = finalize_dt(P, C->dt_bits[step & 1]);
+    const double dtd = clamp_dt(C, ct, finalize_dt(P, C->dt_bits[step & 1]));
 
     const int npx = P.npx, npy = P.npy;
     const long long nv = P.nvol;
@@ -633,7 +633,8 @@ __device__ __forceinline__ void advance_one(Ctrl* C, const DevParams& P)
     const long long s = C->step;
     if (!(t < C->stop_time) || s >= C->stop_step) return;
     const double dt = finalize_dt(P, C->dt_bits[s & 1]);
-    C->t = t + dt;
+    // a clamped step (see clamp_dt) ends exactly on stop_time
+    C->t = (C->clamp && !(dt < C->stop_time - t)) ? C->stop_time : t + dt;
     C->dt_bits[s & 1] = kInfBits;
     C->step = s + 1;
 }

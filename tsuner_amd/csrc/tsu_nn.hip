@@ -1,0 +1,614 @@
+// tsu_nn.hip -- BF16 MFMA kernels of the learned coarse-grid correction.
+//
+// Network: four 3x3 'same' convolutions (8 -> 32 -> 32 -> 32 -> 4 channels),
+// ReLU after the first three.  Activations are NHWC BF16, weights BF16 with
+// FP32 accumulation on v_mfma_f32_16x16x32_bf16, master weights and
+// optimiser state FP32.
+//
+// Every convolution is computed transposed, D^T[m][p] = sum_k Wt[m][k] X^T[k][p]
+// with m = output channel and p = pixel, so that both MFMA operands are
+// 16-byte vectors along the channel axis of an NHWC row (weights [tap][m][k],
+// activations [pixel][k]) and each lane's four results are four consecutive
+// output channels of one pixel (one 8-byte store).  The K axis of an MFMA
+// (32 values) is four "octets" of 8 channels; octet o of the flattened
+// (tap, channel) axis is tap o / (K/8), channels 8*(o % (K/8)) + 0..7, so an
+// 8-channel layer packs four taps into one MFMA and a 32-channel layer one
+// tap.  The data-gradient is the same kernel with the tap offsets negated and
+// the per-tap transposed weights; the weight gradient contracts over pixels.
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "tsu_nn.h"
+
+namespace tsu {
+namespace nn {
+
+typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
+typedef float f32x4 __attribute__((ext_vector_type(4)));
+
+__device__ __forceinline__ uint16_t to_bf16(float f)
+{
+    // round to nearest, ties to even (finite inputs)
+    uint32_t u = __float_as_uint(f);
+    u += 0x7FFFu + ((u >> 16) & 1u);
+    return (uint16_t)(u >> 16);
+}
+
+__device__ __forceinline__ float from_bf16(uint16_t b) { return __uint_as_float((uint32_t)b << 16); }
+
+__device__ __forceinline__ bf16x8 as_bf16x8(uint4 v) { return __builtin_bit_cast(bf16x8, v); }
+
+__device__ __forceinline__ int tap_dy(int t) { return t / 3 - 1; }
+__device__ __forceinline__ int tap_dx(int t) { return t % 3 - 1; }
+
+// out[p][m] = post( sum_t sum_k in[p + SIGN*off(t)][k] * Wt[t][m][k] )
+//   K: input channels (8 or 32), M: output channels held by the weights
+//   (multiple of 16), MOUT: channels stored per output pixel.
+//   POST 0: + bias, ReLU, BF16 out.  1: + bias, FP32 out.  2: zero where
+//   act[p][m] == 0 (ReLU derivative), BF16 out.
+// One wave computes 64 consecutive pixels of one image row; 4 waves per block.
+template <int K, int M, int MOUT, int SIGN, int POST>
+__global__ __launch_bounds__(256) void conv_kernel(const uint16_t* __restrict__ in, const uint16_t* __restrict__ Wt,
+                                                   const float* __restrict__ bias, const uint16_t* __restrict__ act,
+                                                   void* __restrict__ out, int N, int H, int W)
+{
+    constexpr int KO = K / 8;                  // octets per tap
+    constexpr int STEPS = (9 * KO + 3) / 4;    // MFMAs per (m-tile, n-tile)
+    constexpr int MT = M / 16;
+    const int lane = threadIdx.x & 63;
+    const int segs = (W + 63) / 64;
+    const long long wave = (long long)blockIdx.x * 4 + (threadIdx.x >> 6);
+    if (wave >= (long long)N * H * segs) return;
+    const int seg = (int)(wave % segs);
+    const long long row = wave / segs;         // n * H + r
+    const int r = (int)(row % H);
+    const int c0 = seg * 64;
+    const int g = lane >> 4, li = lane & 15;
+
+    f32x4 acc[MT][4];
+#pragma unroll
+    for (int a = 0; a < MT; a++)
+#pragma unroll
+        for (int b = 0; b < 4; b++) acc[a][b] = f32x4{0.f, 0.f, 0.f, 0.f};
+
+#pragma unroll
+    for (int s = 0; s < STEPS; s++) {
+        const int o = 4 * s + g;               // this lane's octet
+        const int t = o / KO, q = o % KO;
+        const bool tv = t < 9;
+        const int dy = SIGN * tap_dy(t), dx = SIGN * tap_dx(t);
+        const int rr = r + dy;
+        const bool rv = tv && rr >= 0 && rr < H;
+        bf16x8 A[MT];
+#pragma unroll
+        for (int a = 0; a < MT; a++) {
+            uint4 v = make_uint4(0, 0, 0, 0);
+            if (tv) v = *(const uint4*)(Wt + ((size_t)(t * M + a * 16 + li) * K + 8 * q));
+            A[a] = as_bf16x8(v);
+        }
+        const uint16_t* base = in + ((size_t)(row + dy) * W) * K + 8 * q;
+#pragma unroll
+        for (int b = 0; b < 4; b++) {
+            const int cc = c0 + b * 16 + li + dx;
+            uint4 v = make_uint4(0, 0, 0, 0);
+            if (rv && cc >= 0 && cc < W) v = *(const uint4*)(base + (size_t)cc * K);
+            const bf16x8 B = as_bf16x8(v);
+#pragma unroll
+            for (int a = 0; a < MT; a++)
+                acc[a][b] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(A[a], B, acc[a][b], 0, 0, 0);
+        }
+    }
+
+#pragma unroll
+    for (int b = 0; b < 4; b++) {
+        const int c = c0 + b * 16 + li;
+        if (c >= W) continue;
+        const size_t p = (size_t)row * W + c;
+#pragma unroll
+        for (int a = 0; a < MT; a++) {
+            const int m0 = a * 16 + 4 * g;
+            if (m0 >= MOUT) continue;
+            float v[4];
+#pragma unroll
+            for (int i = 0; i < 4; i++) v[i] = acc[a][b][i];
+            if (POST == 0 || POST == 1) {
+#pragma unroll
+                for (int i = 0; i < 4; i++) v[i] += bias[m0 + i];
+            }
+            if (POST == 0) {
+#pragma unroll
+                for (int i = 0; i < 4; i++) v[i] = fmaxf(v[i], 0.f);
+            }
+            if (POST == 1) {
+                *(float4*)((float*)out + p * MOUT + m0) = make_float4(v[0], v[1], v[2], v[3]);
+            } else {
+                if (POST == 2) {
+                    const uint2 av = *(const uint2*)(act + p * MOUT + m0);
+                    // ReLU outputs are >= 0: only +0 / -0 bit patterns mean inactive
+                    if ((av.x & 0x7FFFu) == 0) v[0] = 0.f;
+                    if ((av.x & 0x7FFF0000u) == 0) v[1] = 0.f;
+                    if ((av.y & 0x7FFFu) == 0) v[2] = 0.f;
+                    if ((av.y & 0x7FFF0000u) == 0) v[3] = 0.f;
+                }
+                uint2 st;
+                st.x = (uint32_t)to_bf16(v[0]) | ((uint32_t)to_bf16(v[1]) << 16);
+                st.y = (uint32_t)to_bf16(v[2]) | ((uint32_t)to_bf16(v[3]) << 16);
+                *(uint2*)((uint16_t*)out + p * MOUT + m0) = st;
+            }
+        }
+    }
+}
+
+// Weight gradient of out = conv(x, w): dW[t][m][k] = sum_p dy[p][m] x[p + off(t)][k],
+// contracted over pixels on the MFMA K axis (32 pixels per MFMA: four runs of
+// 8 consecutive columns of one row).  Grid: (WAVES/4, 9 taps); wave w of tap t
+// takes rows w, w + WAVES, ... and writes its partial sums to
+// part[t][w][MY][KX]; the centre tap also writes column sums of dy (the bias
+// gradient) to bpart[w][MY].  The reduction kernel adds the partials in a
+// fixed order, so the result is deterministic.
+template <int KX, int MY>
+__global__ __launch_bounds__(256) void wgrad_kernel(const uint16_t* __restrict__ x, const uint16_t* __restrict__ dy,
+                                                    float* __restrict__ part, float* __restrict__ bpart, int N, int H,
+                                                    int W, int waves)
+{
+    constexpr int MT = (MY + 15) / 16, NT = (KX + 15) / 16;
+    const int lane = threadIdx.x & 63;
+    const int w = blockIdx.x * 4 + (threadIdx.x >> 6);
+    const int t = blockIdx.y;
+    if (w >= waves) return;
+    const int g = lane >> 4, li = lane & 15;
+    const int ty = tap_dy(t), tx = tap_dx(t);
+    f32x4 acc[MT][NT];
+#pragma unroll
+    for (int a = 0; a < MT; a++)
+#pragma unroll
+        for (int b = 0; b < NT; b++) acc[a][b] = f32x4{0.f, 0.f, 0.f, 0.f};
+    float bsum[MT];
+#pragma unroll
+    for (int a = 0; a < MT; a++) bsum[a] = 0.f;
+
+    const long long rows = (long long)N * H;
+    for (long long row = w; row < rows; row += waves) {
+        const int r = (int)(row % H);
+        const int rr = r + ty;
+        const bool rv = rr >= 0 && rr < H;
+        for (int c0 = 0; c0 < W; c0 += 32) {
+            const int cb = c0 + 8 * g;          // this lane's 8 pixels: columns cb .. cb + 7
+            bf16x8 A[MT], B[NT];
+#pragma unroll
+            for (int a = 0; a < MT; a++) {
+                const int m = a * 16 + li;
+                uint16_t v[8];
+#pragma unroll
+                for (int j = 0; j < 8; j++) {
+                    const int c = cb + j;
+                    v[j] = (m < MY && c < W) ? dy[((size_t)row * W + c) * MY + m] : (uint16_t)0;
+                }
+                uint4 u;
+                u.x = v[0] | ((uint32_t)v[1] << 16);
+                u.y = v[2] | ((uint32_t)v[3] << 16);
+                u.z = v[4] | ((uint32_t)v[5] << 16);
+                u.w = v[6] | ((uint32_t)v[7] << 16);
+                A[a] = as_bf16x8(u);
+                if (t == 4) {
+#pragma unroll
+                    for (int j = 0; j < 8; j++) bsum[a] += from_bf16(v[j]);
+                }
+            }
+#pragma unroll
+            for (int b = 0; b < NT; b++) {
+                const int k = b * 16 + li;
+                uint16_t v[8];
+#pragma unroll
+                for (int j = 0; j < 8; j++) {
+                    const int c = cb + j, cs = c + tx;
+                    v[j] = (k < KX && c < W && rv && cs >= 0 && cs < W)
+                               ? x[((size_t)(row + ty) * W + cs) * KX + k] : (uint16_t)0;
+                }
+                uint4 u;
+                u.x = v[0] | ((uint32_t)v[1] << 16);
+                u.y = v[2] | ((uint32_t)v[3] << 16);
+                u.z = v[4] | ((uint32_t)v[5] << 16);
+                u.w = v[6] | ((uint32_t)v[7] << 16);
+                B[b] = as_bf16x8(u);
+            }
+#pragma unroll
+            for (int a = 0; a < MT; a++)
+#pragma unroll
+                for (int b = 0; b < NT; b++)
+                    acc[a][b] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(A[a], B[b], acc[a][b], 0, 0, 0);
+        }
+    }
+    // D[m][k]: lane holds m = a*16 + 4*g + i, k = b*16 + li
+    float* dst = part + ((size_t)t * waves + w) * MY * KX;
+#pragma unroll
+    for (int a = 0; a < MT; a++)
+#pragma unroll
+        for (int b = 0; b < NT; b++)
+#pragma unroll
+            for (int i = 0; i < 4; i++) {
+                const int m = a * 16 + 4 * g + i, k = b * 16 + li;
+                if (m < MY && k < KX) dst[m * KX + k] = acc[a][b][i];
+            }
+    if (t == 4) {
+#pragma unroll
+        for (int a = 0; a < MT; a++) {
+            float s = bsum[a];
+            s += __shfl_xor(s, 16);
+            s += __shfl_xor(s, 32);
+            const int m = a * 16 + li;
+            if (g == 0 && m < MY) bpart[(size_t)w * MY + m] = s;
+        }
+    }
+}
+
+// dw[t][m][k] = sum_w part[t][w][m][k] in order w = 0, 1, ...; dw has
+// mrows >= my rows per tap (rows my .. mrows-1 are left untouched).
+__global__ void wgrad_reduce_kernel(const float* __restrict__ part, const float* __restrict__ bpart,
+                                    float* __restrict__ dw, float* __restrict__ db, int my, int kx, int mrows,
+                                    int waves)
+{
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    const int t = blockIdx.y;
+    const int n = my * kx;
+    if (i < n) {
+        float s = 0.f;
+        for (int w = 0; w < waves; w++) s += part[((size_t)t * waves + w) * n + i];
+        dw[((size_t)t * mrows + i / kx) * kx + i % kx] = s;
+    }
+    if (t == 4 && i < my) {
+        float s = 0.f;
+        for (int w = 0; w < waves; w++) s += bpart[(size_t)w * my + i];
+        db[i] = s;
+    }
+}
+
+// Mean-squared-error gradient of the 4-channel FP32 output against the
+// target: dy[p][0..3] = BF16(2 (out - target) / count), dy[p][4..7] = 0; the
+// per-block partial sums of squared error go to loss_part (FP64).
+__global__ __launch_bounds__(256) void mse_kernel(const float* __restrict__ out, const float* __restrict__ tgt,
+                                                  uint16_t* __restrict__ dy, double* __restrict__ loss_part,
+                                                  long long npix, float scale)
+{
+    __shared__ double sh[4];
+    double acc = 0.0;
+    for (long long p = blockIdx.x * (long long)blockDim.x + threadIdx.x; p < npix;
+         p += (long long)gridDim.x * blockDim.x) {
+        const float4 o = *(const float4*)(out + 4 * p);
+        const float4 y = *(const float4*)(tgt + 4 * p);
+        const float d0 = o.x - y.x, d1 = o.y - y.y, d2 = o.z - y.z, d3 = o.w - y.w;
+        acc += (double)d0 * d0 + (double)d1 * d1 + (double)d2 * d2 + (double)d3 * d3;
+        uint4 st;
+        st.x = (uint32_t)to_bf16(scale * d0) | ((uint32_t)to_bf16(scale * d1) << 16);
+        st.y = (uint32_t)to_bf16(scale * d2) | ((uint32_t)to_bf16(scale * d3) << 16);
+        st.z = 0;
+        st.w = 0;
+        *(uint4*)(dy + 8 * p) = st;
+    }
+    for (int off = 32; off > 0; off >>= 1) acc += __shfl_xor(acc, off);
+    if ((threadIdx.x & 63) == 0) sh[threadIdx.x >> 6] = acc;
+    __syncthreads();
+    if (threadIdx.x == 0) loss_part[blockIdx.x] = sh[0] + sh[1] + sh[2] + sh[3];
+}
+
+// Adam (Kingma & Ba 2015) with bias correction folded into the step size.
+__global__ void adam_kernel(float* __restrict__ p, const float* __restrict__ gr, float* __restrict__ m,
+                            float* __restrict__ v, long long n, float lr_t, float b1, float b2, float eps)
+{
+    for (long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x; i < n; i += (long long)gridDim.x * blockDim.x) {
+        const float gi = gr[i];
+        const float mi = b1 * m[i] + (1.f - b1) * gi;
+        const float vi = b2 * v[i] + (1.f - b2) * gi * gi;
+        m[i] = mi;
+        v[i] = vi;
+        p[i] -= lr_t * mi / (sqrtf(vi) + eps);
+    }
+}
+
+// FP32 master weights [9][m][k] -> BF16 forward weights (same layout) and,
+// if wt != null, BF16 per-tap transposes wt[9][k][mt] keeping m < mt.
+__global__ void pack_kernel(const float* __restrict__ w, uint16_t* __restrict__ wb, uint16_t* __restrict__ wt, int m,
+                            int k, int mt)
+{
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= 9 * m * k) return;
+    const int t = i / (m * k), rem = i % (m * k), mi = rem / k, ki = rem % k;
+    const uint16_t b = to_bf16(w[i]);
+    wb[i] = b;
+    if (wt && mi < mt) wt[((size_t)t * k + ki) * mt + mi] = b;
+}
+
+// ---- ensemble-side kernels -------------------------------------------------
+// State planes: h, qx, qy, H, hc (stride nvol); FP32 states add the low
+// words of h, H, hc in planes 5, 6, 7.
+
+template <typename S, bool COMP>
+__device__ __forceinline__ double plane(const S* V, int f, long long nv, long long p)
+{
+    double v = (double)V[f * nv + p];
+    if (COMP && (f == 0 || f == 3 || f == 4)) v += (double)V[(f == 0 ? 5 : f == 3 ? 6 : 7) * nv + p];
+    return v;
+}
+
+template <typename S, bool COMP>
+__device__ __forceinline__ void store_plane(S* V, int f, long long nv, long long p, double v)
+{
+    if (COMP && (f == 0 || f == 3 || f == 4)) {
+        const float hi = (float)v;
+        V[f * nv + p] = (S)hi;
+        V[(f == 0 ? 5 : f == 3 ? 6 : 7) * nv + p] = (S)(float)(v - (double)hi);
+    } else {
+        V[f * nv + p] = (S)v;
+    }
+}
+
+template <typename S, bool COMP>
+__global__ __launch_bounds__(256) void features_kernel(const StageArgs* __restrict__ args, int npx, int npy,
+                                                       NnScales sc, uint16_t* __restrict__ x)
+{
+    const int i = blockIdx.y;
+    const S* V = (const S*)args[i].out;
+    const double dx = args[i].P->dx, dy = args[i].P->dy;
+    const long long nv = (long long)npx * npy;
+    for (long long p = blockIdx.x * (long long)blockDim.x + threadIdx.x; p < nv; p += (long long)gridDim.x * blockDim.x) {
+        const int r = (int)(p / npx), c = (int)(p % npx);
+        const double h = plane<S, COMP>(V, 0, nv, p), H = plane<S, COMP>(V, 3, nv, p);
+        const long long pl = c > 0 ? p - 1 : p, pr = c < npx - 1 ? p + 1 : p;
+        const long long pd = r > 0 ? p - npx : p, pu = r < npy - 1 ? p + npx : p;
+        const double sx = (double)((c > 0) + (c < npx - 1)) * dx, sy = (double)((r > 0) + (r < npy - 1)) * dy;
+        const double dzdx = sx > 0 ? (plane<S, COMP>(V, 3, nv, pl) - plane<S, COMP>(V, 3, nv, pr)) / sx : 0.0;
+        const double dzdy = sy > 0 ? (plane<S, COMP>(V, 3, nv, pd) - plane<S, COMP>(V, 3, nv, pu)) / sy : 0.0;
+        const bool wet = h > sc.wet_h;
+        float f[8];
+        f[0] = (float)h * sc.in[0];
+        f[1] = (float)V[nv + p] * sc.in[1];
+        f[2] = (float)V[2 * nv + p] * sc.in[2];
+        f[3] = (float)(-H) * sc.in[3];
+        f[4] = (float)dzdx * sc.in[4];
+        f[5] = (float)dzdy * sc.in[5];
+        f[6] = wet ? (float)(h - H) * sc.in[6] : 0.f;
+        f[7] = wet ? sc.in[7] : 0.f;
+        uint4 st;
+        st.x = (uint32_t)to_bf16(f[0]) | ((uint32_t)to_bf16(f[1]) << 16);
+        st.y = (uint32_t)to_bf16(f[2]) | ((uint32_t)to_bf16(f[3]) << 16);
+        st.z = (uint32_t)to_bf16(f[4]) | ((uint32_t)to_bf16(f[5]) << 16);
+        st.w = (uint32_t)to_bf16(f[6]) | ((uint32_t)to_bf16(f[7]) << 16);
+        *(uint4*)(x + ((size_t)i * nv + p) * 8) = st;
+    }
+}
+
+template <typename S, bool COMP>
+__global__ __launch_bounds__(256) void apply_kernel(const StageArgs* __restrict__ args, int npx, int npy, NnScales sc,
+                                                    const float* __restrict__ out, long long expect,
+                                                    float* __restrict__ resid)
+{
+    const int i = blockIdx.y;
+    const StageArgs& a = args[i];
+    if (a.ctrl->step != expect) return;
+    S* V = (S*)a.out;
+    const double eps_h = a.P->epsilon_h;
+    const long long nv = (long long)npx * npy;
+    float rmax = 0.f;
+    for (long long p = blockIdx.x * (long long)blockDim.x + threadIdx.x; p < nv; p += (long long)gridDim.x * blockDim.x) {
+        const float4 o = *(const float4*)(out + ((size_t)i * nv + p) * 4);
+        const double h = plane<S, COMP>(V, 0, nv, p), H = plane<S, COMP>(V, 3, nv, p);
+        // residual indicator: largest correction of the surface (h - H)
+        rmax = fmaxf(rmax, fabsf(o.x * sc.out[0] - o.w * sc.out[3]));
+        double h2 = h + (double)(o.x * sc.out[0]);
+        // a dry cell is wetted only by a correction deeper than the wet threshold
+        if (!(h2 > 0.0) || (h <= 0.0 && h2 < sc.wet_h)) h2 = 0.0;
+        const double H2 = H + (double)(o.w * sc.out[3]);
+        double qx = 0.0, qy = 0.0;
+        if (h2 > 0.0) {
+            qx = (double)V[nv + p] + (double)(o.y * sc.out[1]);
+            qy = (double)V[2 * nv + p] + (double)(o.z * sc.out[2]);
+        }
+        store_plane<S, COMP>(V, 0, nv, p, h2);
+        store_plane<S, COMP>(V, 1, nv, p, qx);
+        store_plane<S, COMP>(V, 2, nv, p, qy);
+        store_plane<S, COMP>(V, 3, nv, p, H2);
+        if (h2 == 0.0) store_plane<S, COMP>(V, 4, nv, p, 0.0);
+        if (a.ext) {
+            double* hmax = a.ext;
+            double* bmax = a.ext + nv;
+            double* emax = a.ext + 2 * nv;
+            if (h2 > eps_h) {
+                hmax[p] = fmax(hmax[p], h2);
+                emax[p] = fmax(emax[p], h2 - H2);
+            }
+            bmax[p] = fmax(bmax[p], H2);
+        }
+    }
+    if (resid) {
+        // non-negative floats order like their bit patterns
+        for (int off = 32; off > 0; off >>= 1) rmax = fmaxf(rmax, __shfl_xor(rmax, off));
+        if ((threadIdx.x & 63) == 0 && rmax > 0.f) atomicMax((unsigned int*)(resid + i), __float_as_uint(rmax));
+    }
+}
+
+template <typename S, bool COMP>
+__global__ __launch_bounds__(256) void restrict_kernel(const StageArgs* __restrict__ fine,
+                                                       const StageArgs* __restrict__ coarse, int npx, int npy)
+{
+    const int i = blockIdx.y;
+    const S* F = (const S*)fine[i].out;
+    S* Cs = (S*)coarse[i].out;
+    const long long nv = (long long)npx * npy, fnv = 16 * nv;
+    const int fnx = 4 * npx;
+    for (long long p = blockIdx.x * (long long)blockDim.x + threadIdx.x; p < nv; p += (long long)gridDim.x * blockDim.x) {
+        const int r = (int)(p / npx), c = (int)(p % npx);
+        for (int f = 0; f < 5; f++) {
+            double s = 0.0;
+            for (int dr = 0; dr < 4; dr++)
+                for (int dc = 0; dc < 4; dc++)
+                    s += plane<S, COMP>(F, f, fnv, (long long)(4 * r + dr) * fnx + 4 * c + dc);
+            store_plane<S, COMP>(Cs, f, nv, p, s * 0.0625);
+        }
+    }
+}
+
+template <typename S, bool COMP>
+__global__ __launch_bounds__(256) void target_kernel(const StageArgs* __restrict__ fine,
+                                                     const StageArgs* __restrict__ coarse, int npx, int npy,
+                                                     NnScales sc, float* __restrict__ target)
+{
+    const int i = blockIdx.y;
+    const S* F = (const S*)fine[i].out;
+    const S* Cs = (const S*)coarse[i].out;
+    const long long nv = (long long)npx * npy, fnv = 16 * nv;
+    const int fnx = 4 * npx;
+    const int fields[4] = {0, 1, 2, 3};
+    for (long long p = blockIdx.x * (long long)blockDim.x + threadIdx.x; p < nv; p += (long long)gridDim.x * blockDim.x) {
+        const int r = (int)(p / npx), c = (int)(p % npx);
+        float o[4];
+        for (int j = 0; j < 4; j++) {
+            const int f = fields[j];
+            double s = 0.0;
+            for (int dr = 0; dr < 4; dr++)
+                for (int dc = 0; dc < 4; dc++)
+                    s += plane<S, COMP>(F, f, fnv, (long long)(4 * r + dr) * fnx + 4 * c + dc);
+            o[j] = (float)((s * 0.0625 - plane<S, COMP>(Cs, f, nv, p)) / (double)sc.out[j]);
+        }
+        *(float4*)(target + ((size_t)i * nv + p) * 4) = make_float4(o[0], o[1], o[2], o[3]);
+    }
+}
+
+}  // namespace nn
+
+using namespace nn;
+
+namespace {
+dim3 member_grid(long long nv, int n)
+{
+    long long b = (nv + 255) / 256;
+    if (b > 1024) b = 1024;
+    return dim3((unsigned)b, (unsigned)n);
+}
+}  // namespace
+
+hipError_t nn_launch_features(const StageArgs* args, int n, int npx, int npy, int precision, const NnScales& sc,
+                              uint16_t* x, hipStream_t s)
+{
+    const dim3 g = member_grid((long long)npx * npy, n);
+    if (precision == 0)
+        features_kernel<double, false><<<g, 256, 0, s>>>(args, npx, npy, sc, x);
+    else
+        features_kernel<float, true><<<g, 256, 0, s>>>(args, npx, npy, sc, x);
+    return hipGetLastError();
+}
+
+hipError_t nn_launch_apply(const StageArgs* args, int n, int npx, int npy, int precision, const NnScales& sc,
+                           const float* out, long long expect_step, float* resid, hipStream_t s)
+{
+    const dim3 g = member_grid((long long)npx * npy, n);
+    if (precision == 0)
+        apply_kernel<double, false><<<g, 256, 0, s>>>(args, npx, npy, sc, out, expect_step, resid);
+    else
+        apply_kernel<float, true><<<g, 256, 0, s>>>(args, npx, npy, sc, out, expect_step, resid);
+    return hipGetLastError();
+}
+
+hipError_t nn_launch_restrict(const StageArgs* fine, const StageArgs* coarse, int n, int npx, int npy,
+                              int precision, hipStream_t s)
+{
+    const dim3 g = member_grid((long long)npx * npy, n);
+    if (precision == 0)
+        restrict_kernel<double, false><<<g, 256, 0, s>>>(fine, coarse, npx, npy);
+    else
+        restrict_kernel<float, true><<<g, 256, 0, s>>>(fine, coarse, npx, npy);
+    return hipGetLastError();
+}
+
+hipError_t nn_launch_target(const StageArgs* fine, const StageArgs* coarse, int n, int npx, int npy, int precision,
+                            const NnScales& sc, float* target, hipStream_t s)
+{
+    const dim3 g = member_grid((long long)npx * npy, n);
+    if (precision == 0)
+        target_kernel<double, false><<<g, 256, 0, s>>>(fine, coarse, npx, npy, sc, target);
+    else
+        target_kernel<float, true><<<g, 256, 0, s>>>(fine, coarse, npx, npy, sc, target);
+    return hipGetLastError();
+}
+
+hipError_t nn_launch_conv(int kind, const uint16_t* in, const uint16_t* w, const float* bias, const uint16_t* act,
+                          void* out, int N, int H, int W, hipStream_t s)
+{
+    const long long waves = (long long)N * H * ((W + 63) / 64);
+    const dim3 grid((unsigned)((waves + 3) / 4)), block(256);
+    switch (kind) {
+    case NN_FWD_IN:      // 8 -> 32, ReLU
+        conv_kernel<8, 32, 32, 1, 0><<<grid, block, 0, s>>>(in, w, bias, act, out, N, H, W);
+        break;
+    case NN_FWD_MID:     // 32 -> 32, ReLU
+        conv_kernel<32, 32, 32, 1, 0><<<grid, block, 0, s>>>(in, w, bias, act, out, N, H, W);
+        break;
+    case NN_FWD_OUT:     // 32 -> 4 (weights padded to 16 rows), FP32 out
+        conv_kernel<32, 16, 4, 1, 1><<<grid, block, 0, s>>>(in, w, bias, act, out, N, H, W);
+        break;
+    case NN_DGRAD_OUT:   // dy 8 channels (4 used) -> dx 32, ReLU mask
+        conv_kernel<8, 32, 32, -1, 2><<<grid, block, 0, s>>>(in, w, bias, act, out, N, H, W);
+        break;
+    case NN_DGRAD_MID:   // dy 32 -> dx 32, ReLU mask
+        conv_kernel<32, 32, 32, -1, 2><<<grid, block, 0, s>>>(in, w, bias, act, out, N, H, W);
+        break;
+    default:
+        return hipErrorInvalidValue;
+    }
+    return hipGetLastError();
+}
+
+hipError_t nn_launch_wgrad(int kind, const uint16_t* x, const uint16_t* dy, float* part, float* bpart, float* dw,
+                           float* db, int mrows, int N, int H, int W, hipStream_t s)
+{
+    const int waves = kWgradWaves;
+    const dim3 grid((unsigned)((waves + 3) / 4), 9), block(256);
+    int my, kx;
+    switch (kind) {
+    case NN_WGRAD_IN:
+        wgrad_kernel<8, 32><<<grid, block, 0, s>>>(x, dy, part, bpart, N, H, W, waves);
+        my = 32;
+        kx = 8;
+        break;
+    case NN_WGRAD_MID:
+        wgrad_kernel<32, 32><<<grid, block, 0, s>>>(x, dy, part, bpart, N, H, W, waves);
+        my = 32;
+        kx = 32;
+        break;
+    case NN_WGRAD_OUT:
+        wgrad_kernel<32, 8><<<grid, block, 0, s>>>(x, dy, part, bpart, N, H, W, waves);
+        my = 8;
+        kx = 32;
+        break;
+    default:
+        return hipErrorInvalidValue;
+    }
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess) return e;
+    const int n = my * kx;
+    wgrad_reduce_kernel<<<dim3((n + 255) / 256, 9), 256, 0, s>>>(part, bpart, dw, db, my, kx, mrows, waves);
+    return hipGetLastError();
+}
+
+hipError_t nn_launch_mse(const float* out, const float* tgt, uint16_t* dy, double* loss_part, int nblocks,
+                         long long npix, float scale, hipStream_t s)
+{
+    mse_kernel<<<nblocks, 256, 0, s>>>(out, tgt, dy, loss_part, npix, scale);
+    return hipGetLastError();
+}
+
+hipError_t nn_launch_adam(float* p, const float* g, float* m, float* v, long long n, float lr_t, float b1, float b2,
+                          float eps, hipStream_t s)
+{
+    adam_kernel<<<(unsigned)((n + 255) / 256 < 1024 ? (n + 255) / 256 : 1024), 256, 0, s>>>(p, g, m, v, n, lr_t, b1,
+                                                                                           b2, eps);
+    return hipGetLastError();
+}
+
+hipError_t nn_launch_pack(const float* w, uint16_t* wb, uint16_t* wt, int m, int k, int mt, hipStream_t s)
+{
+    const int n = 9 * m * k;
+    pack_kernel<<<(n + 255) / 256, 256, 0, s>>>(w, wb, wt, m, k, mt);
+    return hipGetLastError();
+}
+
+}  // namespace tsu

@@ -87,6 +87,52 @@ class Ensemble:
     def snapshot(self, restore: bool = False):
         check(lib().tsu_ensemble_snapshot(self._h, int(bool(restore))), "tsu_ensemble_snapshot")
 
+    def set_stop_time(self, t, clamp: bool = False):
+        """Per-member stop times (a scalar applies to all); clamp shortens the
+        crossing step to end exactly on the stop time."""
+        arr = np.ascontiguousarray(np.broadcast_to(np.asarray(t, dtype=np.float64), (self.n,)))
+        check(lib().tsu_ensemble_set_stop_time(self._h, ptr(arr), int(bool(clamp))), "tsu_ensemble_set_stop_time")
+
+    def times(self) -> np.ndarray:
+        t = np.zeros(self.n)
+        check(lib().tsu_ensemble_get_times(self._h, ptr(t), None), "tsu_ensemble_get_times")
+        return t
+
+    def steps_done(self) -> np.ndarray:
+        s = np.zeros(self.n, dtype=np.int64)
+        check(lib().tsu_ensemble_get_times(self._h, None, s.ctypes.data_as(C.POINTER(C.c_longlong))),
+              "tsu_ensemble_get_times")
+        return s
+
+    def extrema(self, i: int):
+        ny, nx = self.shapes[i]
+        a = [np.empty((ny, nx)) for _ in range(3)]
+        check(lib().tsu_ensemble_get_extrema(self._h, int(i), ptr(a[0]), ptr(a[1]), ptr(a[2])),
+              "tsu_ensemble_get_extrema")
+        return a[0], a[1], a[2]
+
+    @property
+    def stream(self) -> int:
+        return lib().tsu_ensemble_stream(self._h) or 0
+
+    # ---- learned-correction hooks (device buffers given as raw addresses) --
+    def nn_features(self, scale_in, wet_h: float, x_addr: int):
+        s = (C.c_float * 8)(*[float(v) for v in scale_in])
+        check(lib().tsu_ensemble_nn_features(self._h, s, C.c_float(wet_h), C.c_void_p(x_addr)),
+              "tsu_ensemble_nn_features")
+
+    def nn_apply(self, scale_out, wet_h: float, out_addr: int, expect_step: int, resid_addr: int = 0):
+        s = (C.c_float * 4)(*[float(v) for v in scale_out])
+        check(lib().tsu_ensemble_nn_apply(self._h, s, C.c_float(wet_h), C.c_void_p(out_addr), int(expect_step),
+                                          C.c_void_p(resid_addr or None)), "tsu_ensemble_nn_apply")
+
+    def restrict_from(self, fine: "Ensemble"):
+        check(lib().tsu_ensemble_restrict(fine._h, self._h), "tsu_ensemble_restrict")
+
+    def nn_target(self, fine: "Ensemble", scale_out, target_addr: int):
+        s = (C.c_float * 4)(*[float(v) for v in scale_out])
+        check(lib().tsu_ensemble_nn_target(fine._h, self._h, s, C.c_void_p(target_addr)), "tsu_ensemble_nn_target")
+
     def time_steps(self, nsteps: int) -> float:
         ms = C.c_double()
         check(lib().tsu_ensemble_time_steps(self._h, int(nsteps), C.byref(ms)), "tsu_ensemble_time_steps")

@@ -107,8 +107,12 @@ class Solver:
     def set_time(self, t: float, dt: float = 0.0):
         check(lib().tsu_set_time(self._h, float(t), float(dt)), "tsu_set_time")
 
-    def set_stop_time(self, t: float):
+    def set_stop_time(self, t: float, clamp: bool = None):
+        """Steps after t are no-ops; clamp=True also shortens the crossing step
+        to end exactly on t (None leaves the clamp setting unchanged)."""
         check(lib().tsu_set_stop_time(self._h, float(t)), "tsu_set_stop_time")
+        if clamp is not None:
+            check(lib().tsu_set_clamp(self._h, int(bool(clamp))), "tsu_set_clamp")
 
     def mass(self):
         out = np.zeros(2)
