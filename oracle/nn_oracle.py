@@ -68,23 +68,34 @@ def conv_weight_grad(x: np.ndarray, dy: np.ndarray):
     return dw, db
 
 
-def forward(x: np.ndarray, params):
-    """params: list of (w, b) for the four layers; returns (output, activations)."""
+def forward(x: np.ndarray, params, rnd=None):
+    """params: list of (w, b) for the four layers; returns (output, activations).
+    rnd, if given, rounds every stored hidden activation (the GPU stores them
+    in BF16)."""
     acts = [x]
     h = x
     for k, (w, b) in enumerate(params):
         z = conv(h, w, b)
-        h = np.maximum(z, 0.0) if k < len(params) - 1 else z
+        if k < len(params) - 1:
+            h = np.maximum(z, 0.0)
+            if rnd is not None:
+                h = rnd(h)
+        else:
+            h = z
         acts.append(h)
     return h, acts
 
 
-def loss_and_grads(x: np.ndarray, target: np.ndarray, params):
-    """Mean-squared error over all outputs and its parameter gradients."""
-    out, acts = forward(x, params)
+def loss_and_grads(x: np.ndarray, target: np.ndarray, params, rnd=None):
+    """Mean-squared error over all outputs and its parameter gradients.  rnd,
+    if given, also rounds the back-propagated signals where the GPU stores
+    them (the output gradient and each masked data gradient)."""
+    out, acts = forward(x, params, rnd)
     diff = out - target
     loss = float(np.mean(diff * diff))
     d = 2.0 * diff / diff.size
+    if rnd is not None:
+        d = rnd(d)
     grads = [None] * len(params)
     for k in range(len(params) - 1, -1, -1):
         w, _ = params[k]
@@ -92,7 +103,15 @@ def loss_and_grads(x: np.ndarray, target: np.ndarray, params):
         grads[k] = (dw, db)
         if k > 0:
             d = conv_input_grad(d, w) * (acts[k] > 0)
+            if rnd is not None:
+                d = rnd(d)
     return loss, grads
+
+
+def bf16_64(a: np.ndarray) -> np.ndarray:
+    """bf16() for float64 arrays (through float32, as the GPU rounds its FP32
+    results), returned as float64."""
+    return bf16(np.asarray(a, dtype=np.float32)).astype(np.float64)
 
 
 def init_params(rng, cin: int = 8, width: int = 32, cout: int = 4):

@@ -30,8 +30,22 @@ def host(t):
     return t.float().cpu().numpy()
 
 
+_KEEP = []
+
+
 def P(t):
+    """Device address of t; t is kept alive until the test ends, so that
+    temporaries passed inline are not recycled by the caching allocator while
+    the enqueued kernels still use them."""
+    _KEEP.append(t)
     return C.c_void_p(t.data_ptr())
+
+
+@pytest.fixture(autouse=True)
+def _release():
+    yield
+    torch.cuda.synchronize()
+    _KEEP.clear()
 
 
 def lib():
@@ -158,10 +172,12 @@ def test_mse_and_adam_kernels():
 
 
 def test_network_gradients_match_oracle():
-    """Full forward + backward on the GPU (BF16 activations) against the FP64
-    oracle's loss_and_grads; the oracle itself is pinned by finite
-    differences in test_nn_oracle.py.  Norm-wise tolerance 3% per tensor for
-    BF16 rounding of activations and gradients."""
+    """Full forward + backward on the GPU against the oracle's loss_and_grads
+    (pinned by finite differences in test_nn_oracle.py), run with BF16
+    rounding at the points where the GPU stores BF16 (hidden activations, the
+    output gradient, the masked data gradients): norm-wise 1% per tensor, the
+    rest being FP32 accumulation and the rare BF16 rounding flips it causes.
+    Against the unrounded FP64 oracle the same gradients agree to 8%."""
     from tsuner_amd import nn
     rng = np.random.default_rng(5)
     n, h, w = 2, 12, 20
@@ -189,18 +205,19 @@ def test_network_gradients_match_oracle():
         if li == 3:
             W, b = W[:, :4, :], b[:4]
         params.append((W, b))
-    loss, grads = O.loss_and_grads(x.astype(np.float64), tgt.astype(np.float64), params)
-    assert abs(gpu_loss - loss) <= 0.02 * loss
+    loss, grads = O.loss_and_grads(x.astype(np.float64), tgt.astype(np.float64), params, rnd=O.bf16_64)
+    loss64, grads64 = O.loss_and_grads(x.astype(np.float64), tgt.astype(np.float64), params)
+    assert abs(gpu_loss - loss) <= 1e-3 * loss
     for li, (m, k) in enumerate(nn.LAYERS):
         wo, bo = nn.OFFS[li]
         gw = g[wo:wo + 9 * m * k].reshape(9, m, k)
         gb = g[bo:bo + m]
-        rw, rb = grads[li]
         if li == 3:
             assert np.all(gw[:, 4:, :] == 0) and np.all(gb[4:] == 0)
             gw, gb = gw[:, :4, :], gb[:4]
-        assert np.linalg.norm(gw - rw) <= 0.03 * np.linalg.norm(rw), li
-        assert np.linalg.norm(gb - rb) <= 0.03 * np.linalg.norm(rb), li
+        for (rw, rb), tol in ((grads[li], 0.01), (grads64[li], 0.08)):
+            assert np.linalg.norm(gw - rw) <= tol * np.linalg.norm(rw), (li, tol)
+            assert np.linalg.norm(gb - rb) <= tol * np.linalg.norm(rb), (li, tol)
 
 
 def test_training_reduces_loss():

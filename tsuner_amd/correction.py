@@ -381,7 +381,7 @@ def main(argv=None):
     result = {}
     if a.cmd in ("train", "all"):
         t0 = time.perf_counter()
-        scale_out = pilot_scales([900001, 900002], 20)
+        scale_out = pilot_scales([900001, 900002], min(20, a.windows))
         log("[train] output scales %s" % scale_out)
         ds = generate(list(range(1, a.seeds + 1)), a.windows, scale_out)
         t_data = time.perf_counter() - t0
