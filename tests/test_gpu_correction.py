@@ -1,0 +1,51 @@
+"""The learned-correction pipeline end to end at a small size (fine 64^2,
+coarse 16^2): training pairs line up in time, a zero network leaves the
+coarse run unchanged bit for bit, and training on the pairs lowers the loss."""
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+torch = pytest.importorskip("torch")
+
+
+def test_pairs_zero_network_and_training():
+    from tsuner_amd import correction as K
+    from tsuner_amd import nn
+    scale = [0.05, 0.5, 0.5, 0.01]
+    ds = K.generate([11, 12], windows=4, scale_out=scale, fine_n=64)
+    assert ds.count == 8
+    x = ds.x.float().cpu().numpy()
+    y = ds.y.cpu().numpy()
+    assert x.shape == (8, 16, 16, 8) and y.shape == (8, 16, 16, 4)
+    assert np.all(np.isfinite(x)) and np.all(np.isfinite(y))
+    assert np.abs(y).max() > 0      # the coarse solver does differ from the restricted fine one
+
+    # a network whose output is identically zero must not change the run
+    fine_sc = [K.fine_scenario(s, extrema=1, n=64) for s in (21, 22)]
+    coarse_sc = [K.coarse_scenario(f) for f in fine_sc]
+    t_end = np.array([30.0, 45.0])
+    zero = nn.Net(np.zeros(nn.NPARAM, np.float32), max_batch=2, h=16, w=16)
+    a = K.run_coarse(coarse_sc, t_end)
+    b = K.run_coarse(coarse_sc, t_end, net=zero, scale_out=scale)
+    assert np.all(a.t_end == t_end) and np.all(b.t_end == t_end)
+    for i in range(2):
+        np.testing.assert_array_equal(np.nan_to_num(a.eta_max[i], nan=-7.0), np.nan_to_num(b.eta_max[i], nan=-7.0))
+        np.testing.assert_array_equal(a.hmax[i], b.hmax[i])
+    assert np.all(b.resid == 0)
+
+    # a few epochs on the pairs lower the loss over the whole set
+    def full_loss(net):
+        tot = 0.0
+        for b in range(2):
+            xa = ds.x.data_ptr() + b * 4 * 16 * 16 * 8 * 2
+            ya = ds.y.data_ptr() + b * 4 * 16 * 16 * 4 * 4
+            net.forward(xa, 4)
+            net.backward(xa, ya, 4)
+            tot += net.loss(4)
+        return tot
+
+    net = nn.Net(nn.init_params(3), max_batch=4, h=16, w=16)
+    before = full_loss(net)
+    net, hist = K.train(ds, epochs=8, batch=4, lr=3e-3, net=net)
+    assert full_loss(net) < 0.7 * before

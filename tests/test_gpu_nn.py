@@ -164,11 +164,14 @@ def test_mse_and_adam_kernels():
     tm, tv = torch.zeros_like(tp), torch.zeros_like(tp)
     ok(lib().tsu_nn_adam(P(tp), P(tg), P(tm), P(tv), 300, C.c_float(1e-2), C.c_float(0.9), C.c_float(0.999),
                          C.c_float(1e-8), C.c_void_p(0)))
-    m = 0.1 * gr.astype(np.float64)
-    v = 0.001 * gr.astype(np.float64) ** 2
+    # the kernel's FP32 constants: (1 - 0.9f) and (1 - 0.999f) are not 0.1 and 0.001
+    one, b1, b2 = np.float32(1), np.float32(0.9), np.float32(0.999)
+    m = (one - b1) * gr
+    v = (one - b2) * gr * gr
     np.testing.assert_allclose(host(tm), m, rtol=1e-6)
     np.testing.assert_allclose(host(tv), v, rtol=1e-6)
-    np.testing.assert_allclose(host(tp), p - 1e-2 * m / (np.sqrt(v) + 1e-8), rtol=1e-5, atol=1e-7)
+    np.testing.assert_allclose(host(tp), p - np.float32(1e-2) * m / (np.sqrt(v) + np.float32(1e-8)), rtol=1e-5,
+                               atol=1e-7)
 
 
 def test_network_gradients_match_oracle():

@@ -10,7 +10,9 @@ import os
 import numpy as np
 
 HERE = os.path.dirname(os.path.abspath(__file__))
-LIB_PATH = os.path.join(HERE, "libtsuner.so")
+# TSUNER_AMD_LIB selects another build of the library (e.g. an occupancy
+# variant from build.build_variant); default: the in-tree build.
+LIB_PATH = os.environ.get("TSUNER_AMD_LIB") or os.path.join(HERE, "libtsuner.so")
 
 TSU_FP64, TSU_FP32 = 0, 1
 STATUS = {0: "ok", 1: "device memory", 2: "host", 3: "argument", 4: "hip", 5: "unsupported"}

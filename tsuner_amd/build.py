@@ -29,13 +29,28 @@ DEVICE = ["-x", "hip", "--offload-arch=gfx950", "-Xclang", "-target-feature", "-
           "-mcumode"]
 
 
-def _compile(src: str) -> str:
-    obj = os.path.join(BUILD, os.path.splitext(src)[0] + ".o")
-    cmd = [HIPCC] + COMMON + DEVICE + ["-c", os.path.join(CSRC, src), "-o", obj]
+def _compile(src: str, defines=(), build_dir: str = BUILD) -> str:
+    obj = os.path.join(build_dir, os.path.splitext(src)[0] + ".o")
+    cmd = [HIPCC] + COMMON + DEVICE + ["-D" + d for d in defines] + ["-c", os.path.join(CSRC, src), "-o", obj]
     r = subprocess.run(cmd, capture_output=True, text=True)
     if r.returncode != 0:
         raise RuntimeError("compile failed: %s\n%s\n%s" % (" ".join(cmd), r.stdout, r.stderr))
     return obj
+
+
+def build_variant(out: str, defines) -> str:
+    """The library alone, compiled with extra -D defines (e.g. the stage
+    kernels' TSU_WAVES_FP64 / TSU_WAVES_FP32 occupancy targets) into `out`;
+    load it with TSUNER_AMD_LIB=out."""
+    bdir = os.path.join(BUILD, "variant_" + "_".join(d.replace("=", "") for d in defines))
+    os.makedirs(bdir, exist_ok=True)
+    with cf.ThreadPoolExecutor(max_workers=4) as ex:
+        objs = list(ex.map(lambda s: _compile(s, defines, bdir), SOURCES))
+    cmd = [HIPCC, "--offload-arch=gfx950", "-shared", "-fPIC", "-o", out] + objs
+    r = subprocess.run(cmd, capture_output=True, text=True)
+    if r.returncode != 0:
+        raise RuntimeError("link failed: %s\n%s\n%s" % (" ".join(cmd), r.stdout, r.stderr))
+    return out
 
 
 def build(verbose: bool = False) -> str:

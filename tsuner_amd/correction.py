@@ -57,8 +57,8 @@ def coarse_scenario(fine: S.Scenario) -> S.Scenario:
     return S.Scenario(fine.name + "_coarse", p, dict(fine.scheme), np.ascontiguousarray(V), fix, steps=fine.steps)
 
 
-def fine_scenario(seed: int, extrema: int = 0) -> S.Scenario:
-    return S.gaussian_source(FINE_N, FINE_DX, seed=seed, extrema=extrema)
+def fine_scenario(seed: int, extrema: int = 0, n: int = FINE_N) -> S.Scenario:
+    return S.gaussian_source(n, FINE_DX, seed=seed, extrema=extrema)
 
 
 class _Stream:
@@ -99,16 +99,16 @@ class Dataset:
 
 
 def generate(seeds: Sequence[int], windows: int, scale_out: Sequence[float], precision: str = "fp32",
-             capacity: int = None, seed_perm: int = 0) -> Dataset:
+             capacity: int = None, seed_perm: int = 0, fine_n: int = FINE_N) -> Dataset:
     """Training pairs from fine/coarse runs of `seeds`, `windows` windows each.
     Pairs are written in a random order (contiguous batches are then random)."""
     import torch
     from .ensemble import Ensemble
-    fine_sc = [fine_scenario(s) for s in seeds]
+    fine_sc = [fine_scenario(s, n=fine_n) for s in seeds]
     coarse_sc = [coarse_scenario(f) for f in fine_sc]
     m = len(seeds)
     total = m * windows if capacity is None else capacity
-    npx = COARSE_N
+    npx = fine_n // FACTOR
     X = torch.zeros((total, npx, npx, 8), dtype=torch.bfloat16, device="cuda")
     Y = torch.zeros((total, npx, npx, 4), dtype=torch.float32, device="cuda")
     perm = np.random.default_rng(seed_perm).permutation(total)
@@ -142,12 +142,17 @@ def generate(seeds: Sequence[int], windows: int, scale_out: Sequence[float], pre
     return Dataset(X, Y, list(scale_out), min(k, total), list(seeds))
 
 
-def pilot_scales(seeds: Sequence[int], windows: int) -> List[float]:
-    """RMS of the raw corrections (h, qx, qy, H) over a short pilot run."""
-    ds = generate(seeds, windows, [1.0, 1.0, 1.0, 1.0])
+# smallest output scales: 1 mm of depth, 0.01 m^2/s of discharge, 0.1 mm of bed
+SCALE_OUT_FLOOR = [1e-3, 1e-2, 1e-2, 1e-4]
+
+
+def pilot_scales(seeds: Sequence[int], windows: int, fine_n: int = FINE_N) -> List[float]:
+    """RMS of the raw corrections (h, qx, qy, H) over a short pilot run, not
+    below SCALE_OUT_FLOOR."""
+    ds = generate(seeds, windows, [1.0, 1.0, 1.0, 1.0], fine_n=fine_n)
     y = ds.y[:ds.count].cpu().numpy().reshape(-1, 4).astype(np.float64)
     rms = np.sqrt(np.mean(y * y, axis=0))
-    return [float(max(r, 1e-6)) for r in rms]
+    return [float(max(r, f)) for r, f in zip(rms, SCALE_OUT_FLOOR)]
 
 
 def train(ds: Dataset, epochs: int, batch: int = 8, lr: float = 2e-3, seed: int = 0, net: "nn.Net" = None,
@@ -310,12 +315,12 @@ def load(path: str = WEIGHTS):
         return f["params"].astype(np.float32), [float(v) for v in f["scale_out"]], json.loads(str(f["info"]))
 
 
-def evaluate(net, scale_out, seeds: Sequence[int], fine_steps: int = 3000, resid_tau: float = 0.5,
+def evaluate(net, scale_out, seeds: Sequence[int], fine_steps: int = 3000, resid_tau: float = 0.5, fine_n: int = FINE_N,
              max_batch: int = 32) -> dict:
     """Held-out evaluation: fine members (fine_steps steps) against the coarse
     solver alone and coarse + network, with the residual fallback."""
     t0 = time.perf_counter()
-    fine_sc = [fine_scenario(s, extrema=1) for s in seeds]
+    fine_sc = [fine_scenario(s, extrema=1, n=fine_n) for s in seeds]
     coarse_sc = [coarse_scenario(f) for f in fine_sc]
     log("[eval] %d scenarios generated (%.1f s)" % (len(seeds), time.perf_counter() - t0))
     fine = run_fine(fine_sc, fine_steps)
@@ -338,12 +343,20 @@ def evaluate(net, scale_out, seeds: Sequence[int], fine_steps: int = 3000, resid
         ru = np.array([0.0 if (use_fine is not None and use_fine[i]) else m["runup_rel_err"] for i, m in enumerate(ms)])
         ok = np.isfinite(pe)
         okr = np.isfinite(ru)
-        return {"peak_eta_err_mean_m": float(np.mean(pe[ok])), "peak_eta_err_p95_m": float(np.percentile(pe[ok], 95)),
-                "peak_eta_err_max_m": float(np.max(pe[ok])),
-                "nearshore_mean_abs_eta_err_m": float(np.nanmean(me)),
-                "runup_rel_err_mean": float(np.mean(ru[okr])), "runup_rel_err_p95": float(np.percentile(ru[okr], 95)),
-                "frac_peak_eta_within_5cm": float(np.mean(pe[ok] <= 0.05)),
-                "frac_runup_within_10pct": float(np.mean(ru[okr] <= 0.10))}
+
+        def stat(f, a):
+            return float(f(a)) if a.size else None
+
+        return {"peak_eta_err_mean_m": stat(np.mean, pe[ok]),
+                "peak_eta_err_p95_m": stat(lambda a: np.percentile(a, 95), pe[ok]),
+                "peak_eta_err_max_m": stat(np.max, pe[ok]),
+                "nearshore_mean_abs_eta_err_m": stat(np.mean, me[np.isfinite(me)]),
+                "runup_rel_err_mean": stat(np.mean, ru[okr]),
+                "runup_rel_err_p95": stat(lambda a: np.percentile(a, 95), ru[okr]),
+                "frac_peak_eta_within_5cm": stat(np.mean, (pe[ok] <= 0.05).astype(float)),
+                "frac_runup_within_10pct": stat(np.mean, (ru[okr] <= 0.10).astype(float)),
+                "scenarios_with_runup": int(sum(m["runup_fine"] > 0 for m in ms)),
+                "scenarios_with_peak": int(ok.sum())}
 
     out = {
         "scenarios": n, "fine_steps": fine_steps,

@@ -58,7 +58,8 @@ struct tsu_ensemble {
     Ctrl* ctrl = nullptr;
     Ctrl* snapC = nullptr;
     DevParams* dP = nullptr;
-    tsu::StageArgs* dargs = nullptr;    // [2][n]: stage 0 and stage 1 arguments
+    tsu::StageArgs* dargs = nullptr;    // [2][n]: stage 0 and stage 1 arguments (hooks)
+    tsu::BatchMember* dmem = nullptr;   // [n]: members as the batched stage kernel sees them
     int2* tiles = nullptr;
     int ntiles = 0;
     double* mass_buf = nullptr;
@@ -107,6 +108,7 @@ void destroy(tsu_ensemble* e)
     (void)hipFree(e->snapC);
     (void)hipFree(e->dP);
     (void)hipFree(e->dargs);
+    (void)hipFree(e->dmem);
     (void)hipFree(e->tiles);
     (void)hipFree(e->mass_buf);
     if (e->stream) (void)hipStreamDestroy(e->stream);
@@ -121,9 +123,19 @@ int nchunks(const tsu_ensemble* e, int i) { return (e->p[i].npy + e->rows[i] - 1
 int build_launch_tables(tsu_ensemble* e)
 {
     std::vector<tsu::StageArgs> args(2 * (size_t)e->n);
+    std::vector<tsu::BatchMember> mem((size_t)e->n);
     std::vector<int2> tiles;
     for (int i = 0; i < e->n; i++) {
         const size_t plane = (size_t)e->P[i].nvol * e->esize;
+        tsu::BatchMember& bm = mem[(size_t)i];
+        bm.off = (long long)e->off[i];
+        bm.plane = (long long)plane;
+        bm.foff = (long long)e->foff[i];
+        bm.xoff = e->s[i].extrema ? (long long)e->xoff[i] : -1;
+        bm.tile_rows = e->rows[i];
+        bm.nstrips = nstrips(e, i);
+        bm.ntiles = bm.nstrips * nchunks(e, i);
+        bm.order = e->s[i].order;
         for (int k = 0; k < 2; k++) {
             tsu::StageArgs a{};
             char* Ai = e->A + e->off[i];
@@ -162,16 +174,17 @@ int build_launch_tables(tsu_ensemble* e)
     ECHK(hipMalloc((void**)&e->tiles, tiles.size() * sizeof(int2)));
     ECHK(hipMemcpy(e->tiles, tiles.data(), tiles.size() * sizeof(int2), hipMemcpyHostToDevice));
     ECHK(hipMemcpy(e->dargs, args.data(), args.size() * sizeof(tsu::StageArgs), hipMemcpyHostToDevice));
+    ECHK(hipMemcpy(e->dmem, mem.data(), mem.size() * sizeof(tsu::BatchMember), hipMemcpyHostToDevice));
     return TSU_OK;
 }
 
 int enqueue_step(tsu_ensemble* e)
 {
     const int order = e->s[0].order;
-    for (int k = 0; k < order; k++) {
-        const tsu::StageArgs* a = e->dargs + (size_t)k * e->n;
-        ECHK(tsu::launch_stage_batch(a, e->tiles, e->ntiles, k, k == order - 1, e->sed, e->precision, e->stream));
-    }
+    const tsu::BatchPools pools{e->A, e->B, e->fix, e->ctrl, e->ext, e->dP};
+    for (int k = 0; k < order; k++)
+        ECHK(tsu::launch_stage_batch(pools, e->dmem, e->tiles, e->ntiles, k, k == order - 1, e->sed, e->precision,
+                                     e->stream));
     ECHK(tsu::launch_advance_batch(e->ctrl, e->dP, e->n, e->stream));
     return TSU_OK;
 }
@@ -255,6 +268,7 @@ TSU_API int tsu_ensemble_create(tsu_ensemble** out, int n, const tsu_sim_params*
     if (err == hipSuccess) err = hipMalloc((void**)&e->ctrl, n * sizeof(Ctrl));
     if (err == hipSuccess) err = hipMalloc((void**)&e->dP, n * sizeof(DevParams));
     if (err == hipSuccess) err = hipMalloc((void**)&e->dargs, 2 * (size_t)n * sizeof(tsu::StageArgs));
+    if (err == hipSuccess) err = hipMalloc((void**)&e->dmem, (size_t)n * sizeof(tsu::BatchMember));
     if (err == hipSuccess) err = hipMalloc((void**)&e->mass_buf, (2 * (size_t)e->mass_nb + 2) * sizeof(double));
     if (err == hipSuccess) err = hipMemcpy(e->dP, e->P.data(), n * sizeof(DevParams), hipMemcpyHostToDevice);
     if (err != hipSuccess) {

@@ -31,9 +31,29 @@ struct StageArgs {
 
 // precision: 0 = FP64 planes, 1 = FP32 planes (FP64 arithmetic).
 hipError_t launch_stage(const StageArgs& a, bool sed, int precision, hipStream_t s);
+
+// Ensemble member as seen by the batched stage kernel: byte offsets into
+// the ensemble's pools instead of pointers, so that every address the kernel
+// forms derives from a kernel-argument base pointer (global address space:
+// global_load with scalar bases, not flat loads with 64-bit VGPR addresses).
+struct BatchMember {
+    long long off;        // state offset in pools A and B (bytes)
+    long long plane;      // bytes per plane (nvol * element size)
+    long long foff;       // mask offset (bytes)
+    long long xoff;       // extrema offset (doubles), -1: none
+    int tile_rows, nstrips, ntiles, order;
+};
+struct BatchPools {
+    char* A;              // U^n (stage 2 writes it in place)
+    char* B;              // stage-1 result
+    const uint8_t* fix;
+    Ctrl* ctrl;           // one per member
+    double* ext;
+    const DevParams* P;   // one per member
+};
 // Ensemble: one launch over all tiles of all members (tiles[i] = member, tile).
-hipError_t launch_stage_batch(const StageArgs* scen, const int2* tiles, int ntiles, int k, int last, bool sed,
-                              int precision, hipStream_t s);
+hipError_t launch_stage_batch(const BatchPools& pools, const BatchMember* mem, const int2* tiles, int ntiles, int k,
+                              int last, bool sed, int precision, hipStream_t s);
 hipError_t launch_advance_batch(Ctrl* ctrl, const DevParams* P, int n, hipStream_t s);
 hipError_t launch_init_dt(const void* V, const DevParams& P, Ctrl* ctrl, int precision, hipStream_t s);
 hipError_t launch_warmup(hipStream_t s);

@@ -592,8 +592,22 @@ __device__ __forceinline__ void stage_body(const StageArgs& a, const int tile)
     }
 }
 
+// Waves per SIMD the register allocation targets.  Measured on the 8192^2
+// bench: FP32 at 3 waves (167 VGPRs, 4 spilled) runs 24% faster than at 2
+// (173 VGPRs); 4 waves spill 39 VGPRs and lose.  FP64 needs 229 VGPRs; at 3
+// waves it spills 87 and runs at half speed, so it stays at 2.
+#ifndef TSU_WAVES_FP64
+#define TSU_WAVES_FP64 2
+#endif
+#ifndef TSU_WAVES_FP32
+#define TSU_WAVES_FP32 3
+#endif
+template <typename R>
+constexpr int stage_waves() { return sizeof(R) == 8 ? TSU_WAVES_FP64 : TSU_WAVES_FP32; }
+
 template <typename R, typename S, bool SED>
-__global__ __launch_bounds__(64) void stage_kernel(StageArgs a)
+__global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(stage_waves<R>(), stage_waves<R>())))
+void stage_kernel(StageArgs a)
 {
     stage_body<R, S, SED>(a, xcd_tile(blockIdx.x, gridDim.x));
 }
@@ -602,13 +616,38 @@ __global__ __launch_bounds__(64) void stage_kernel(StageArgs a)
 // (scenario, tile within the scenario); scen[s] holds that scenario's
 // buffers, constants and step-control block.
 template <typename R, typename S, bool SED>
-__global__ __launch_bounds__(64) void stage_kernel_batch(const StageArgs* __restrict__ scen,
-                                                         const int2* __restrict__ tiles, int k, int last)
+__global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(stage_waves<R>(), stage_waves<R>())))
+void stage_kernel_batch(BatchPools pools, const BatchMember* __restrict__ mem, const int2* __restrict__ tiles, int k,
+                        int last)
 {
     const int2 e = tiles[xcd_tile(blockIdx.x, gridDim.x)];
-    StageArgs a = scen[e.x];
+    const BatchMember m = mem[e.x];
+    char* Ai = pools.A + m.off;
+    char* Bi = pools.B + m.off;
+    StageArgs a;
+    a.pingpong = 0;
+    if (m.order == 1) {
+        a.in = Ai;
+        a.out = Bi;
+        a.pingpong = 1;
+    } else if (k == 0) {
+        a.in = Ai;
+        a.out = Bi;
+    } else {
+        a.in = Bi;
+        a.out = Ai;
+    }
+    a.v0 = Ai;
+    a.Hin = Ai + 3 * m.plane;
+    a.fix = pools.fix + m.foff;
+    a.ctrl = pools.ctrl + e.x;
+    a.ext = m.xoff >= 0 ? pools.ext + m.xoff : nullptr;
     a.k = k;
     a.last = last;
+    a.tile_rows = m.tile_rows;
+    a.nstrips = m.nstrips;
+    a.ntiles = m.ntiles;
+    a.P = pools.P + e.x;
     stage_body<R, S, SED>(a, e.y);
 }
 
@@ -795,20 +834,22 @@ hipError_t launch_stage(const StageArgs& a, bool sed, int precision, hipStream_t
     return hipGetLastError();
 }
 
-hipError_t launch_stage_batch(const StageArgs* scen, const int2* tiles, int ntiles, int k, int last, bool sed,
-                              int precision, hipStream_t s)
+hipError_t launch_stage_batch(const BatchPools& pools, const BatchMember* mem, const int2* tiles, int ntiles, int k,
+                              int last, bool sed, int precision, hipStream_t s)
 {
     dim3 grid(ntiles), block(64);
     if (precision == 0) {
         if (sed)
-            hipLaunchKernelGGL((stage_kernel_batch<double, double, true>), grid, block, 0, s, scen, tiles, k, last);
+            hipLaunchKernelGGL((stage_kernel_batch<double, double, true>), grid, block, 0, s, pools, mem, tiles, k, last);
         else
-            hipLaunchKernelGGL((stage_kernel_batch<double, double, false>), grid, block, 0, s, scen, tiles, k, last);
+            hipLaunchKernelGGL((stage_kernel_batch<double, double, false>), grid, block, 0, s, pools, mem, tiles, k,
+                               last);
     } else {
         if (sed)
-            hipLaunchKernelGGL((stage_kernel_batch<float, float, true>), grid, block, 0, s, scen, tiles, k, last);
+            hipLaunchKernelGGL((stage_kernel_batch<float, float, true>), grid, block, 0, s, pools, mem, tiles, k, last);
         else
-            hipLaunchKernelGGL((stage_kernel_batch<float, float, false>), grid, block, 0, s, scen, tiles, k, last);
+            hipLaunchKernelGGL((stage_kernel_batch<float, float, false>), grid, block, 0, s, pools, mem, tiles, k,
+                               last);
     }
     return hipGetLastError();
 }
