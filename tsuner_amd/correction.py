@@ -156,11 +156,16 @@ def pilot_scales(seeds: Sequence[int], windows: int, fine_n: int = FINE_N) -> Li
 
 
 def train(ds: Dataset, epochs: int, batch: int = 8, lr: float = 2e-3, seed: int = 0, net: "nn.Net" = None,
-          log_every: int = 1):
-    """Adam over contiguous batches of the (pre-shuffled) dataset; the batch
-    order is reshuffled every epoch.  Returns the network and loss history."""
+          log_every: int = 1, val_frac: float = 0.1):
+    """Adam over contiguous batches of the (pre-shuffled) dataset, the last
+    `val_frac` of the batches held out for validation; the batch order is
+    reshuffled every epoch.  Returns the network and the per-epoch
+    validation losses (mean squared error of the normalised targets; 1.0 is
+    the loss of predicting no correction when the scales are the RMS)."""
     n = ds.count
     nb = n // batch
+    nval = min(int(round(nb * val_frac)), nb - 1) if nb > 1 else 0
+    ntr = nb - nval
     h = ds.x.shape[1]
     if net is None:
         net = nn.Net(nn.init_params(seed), max_batch=batch, h=h, w=h)
@@ -169,21 +174,35 @@ def train(ds: Dataset, epochs: int, batch: int = 8, lr: float = 2e-3, seed: int 
     base = ds.x.data_ptr(), ds.y.data_ptr()
     xstride = h * h * 8 * 2
     ystride = h * h * 4 * 4
+
+    def addr(b):
+        return base[0] + int(b) * batch * xstride, base[1] + int(b) * batch * ystride
+
+    def val_loss():
+        if nval == 0:
+            return float("nan")
+        tot = 0.0
+        for b in range(ntr, nb):
+            xa, ya = addr(b)
+            net.forward(xa, batch)
+            net.backward(xa, ya, batch)
+            tot += net.loss(batch)
+        return tot / nval
+
     t0 = time.perf_counter()
+    hist.append(val_loss())
+    log("[train] before training: validation loss %.4f" % hist[-1])
     for ep in range(epochs):
         lr_ep = lr * 0.5 * (1.0 + np.cos(np.pi * ep / max(1, epochs)))
-        for b in rng.permutation(nb):
-            xa = base[0] + int(b) * batch * xstride
-            ya = base[1] + int(b) * batch * ystride
+        for b in rng.permutation(ntr if ntr > 0 else nb):
+            xa, ya = addr(b)
             net.forward(xa, batch)
             net.backward(xa, ya, batch)
             net.adam(lr_ep)
-        # loss of the epoch's last batch (one synchronisation per epoch)
-        loss = net.loss(batch)
-        hist.append(loss)
-        if ep % log_every == 0:
-            log("[train] epoch %d/%d lr %.2e loss %.4f (%.1f s)" % (ep + 1, epochs, lr_ep, loss,
-                                                                   time.perf_counter() - t0))
+        hist.append(val_loss())
+        if ep % log_every == 0 or ep == epochs - 1:
+            log("[train] epoch %d/%d lr %.2e validation loss %.4f (%.1f s)" % (ep + 1, epochs, lr_ep, hist[-1],
+                                                                              time.perf_counter() - t0))
     return net, hist
 
 
@@ -270,10 +289,14 @@ def run_coarse(scs: Sequence[S.Scenario], t_end: np.ndarray, net: "nn.Net" = Non
 
 
 def _block_mean_nan(a: np.ndarray) -> np.ndarray:
+    """Means over FACTOR x FACTOR blocks ignoring NaN (never-wet) cells; NaN
+    where a whole block is NaN."""
     n = a.shape[0] // FACTOR
     b = a.reshape(n, FACTOR, n, FACTOR)
-    with np.errstate(invalid="ignore"):
-        return np.nanmean(b, axis=(1, 3)) if np.isfinite(b).any() else np.full((n, n), np.nan)
+    cnt = np.isfinite(b).sum(axis=(1, 3))
+    s = np.where(np.isfinite(b), b, 0.0).sum(axis=(1, 3))
+    with np.errstate(invalid="ignore", divide="ignore"):
+        return np.where(cnt > 0, s / np.maximum(cnt, 1), np.nan)
 
 
 def runup(H0: np.ndarray, hmax: np.ndarray) -> float:

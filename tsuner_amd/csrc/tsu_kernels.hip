@@ -50,11 +50,27 @@ __device__ __forceinline__ int xcd_tile(int b, int nb)
     return x * q + min(x, r) + (b >> 3);
 }
 
-__device__ __forceinline__ double up1(double v) { return __shfl_up(v, 1, 64); }
-__device__ __forceinline__ double dn1(double v) { return __shfl_down(v, 1, 64); }
-__device__ __forceinline__ float up1(float v) { return __shfl_up(v, 1, 64); }
-__device__ __forceinline__ float dn1(float v) { return __shfl_down(v, 1, 64); }
-__device__ __forceinline__ int up1i(int v) { return __shfl_up(v, 1, 64); }
+// Whole-wave shifts by one lane as DPP moves (wave_shr:1 gives lane i the
+// value of lane i - 1, wave_shl:1 that of lane i + 1): one VALU move per
+// dword instead of an LDS permute and its wait.  Lanes shifted in from
+// outside the wave read 0; they are halo lanes, whose results are not stored.
+__device__ __forceinline__ int dpp_up1(int v) { return __builtin_amdgcn_update_dpp(0, v, 0x138, 0xF, 0xF, true); }
+__device__ __forceinline__ int dpp_dn1(int v) { return __builtin_amdgcn_update_dpp(0, v, 0x130, 0xF, 0xF, true); }
+__device__ __forceinline__ double up1(double v)
+{
+    const long long x = __double_as_longlong(v);
+    const int lo = dpp_up1((int)x), hi = dpp_up1((int)(x >> 32));
+    return __longlong_as_double(((long long)hi << 32) | (unsigned int)lo);
+}
+__device__ __forceinline__ double dn1(double v)
+{
+    const long long x = __double_as_longlong(v);
+    const int lo = dpp_dn1((int)x), hi = dpp_dn1((int)(x >> 32));
+    return __longlong_as_double(((long long)hi << 32) | (unsigned int)lo);
+}
+__device__ __forceinline__ float up1(float v) { return __int_as_float(dpp_up1(__float_as_int(v))); }
+__device__ __forceinline__ float dn1(float v) { return __int_as_float(dpp_dn1(__float_as_int(v))); }
+__device__ __forceinline__ int up1i(int v) { return dpp_up1(v); }
 
 __device__ __forceinline__ double wave_min(double v)
 {
@@ -593,20 +609,27 @@ __device__ __forceinline__ void stage_body(const StageArgs& a, const int tile)
 }
 
 // Waves per SIMD the register allocation targets.  Measured on the 8192^2
-// bench: FP32 at 3 waves (167 VGPRs, 4 spilled) runs 24% faster than at 2
-// (173 VGPRs); 4 waves spill 39 VGPRs and lose.  FP64 needs 229 VGPRs; at 3
-// waves it spills 87 and runs at half speed, so it stays at 2.
+// bench: FP32 at 3 waves (167 VGPRs, a few spilled) runs 24% faster than at
+// 2 (173 VGPRs); 4 waves spill 39 VGPRs and lose.  FP64 with sediment needs
+// 226 VGPRs; at 3 waves it spills ~90 and runs at half speed, so it stays at
+// 2.  FP64 without sediment fits 168 VGPRs, i.e. 3 waves, unspilled.
 #ifndef TSU_WAVES_FP64
 #define TSU_WAVES_FP64 2
+#endif
+#ifndef TSU_WAVES_FP64_NOSED
+#define TSU_WAVES_FP64_NOSED 3
 #endif
 #ifndef TSU_WAVES_FP32
 #define TSU_WAVES_FP32 3
 #endif
-template <typename R>
-constexpr int stage_waves() { return sizeof(R) == 8 ? TSU_WAVES_FP64 : TSU_WAVES_FP32; }
+template <typename R, bool SED>
+constexpr int stage_waves()
+{
+    return sizeof(R) == 4 ? TSU_WAVES_FP32 : (SED ? TSU_WAVES_FP64 : TSU_WAVES_FP64_NOSED);
+}
 
 template <typename R, typename S, bool SED>
-__global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(stage_waves<R>(), stage_waves<R>())))
+__global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(stage_waves<R, SED>(), stage_waves<R, SED>())))
 void stage_kernel(StageArgs a)
 {
     stage_body<R, S, SED>(a, xcd_tile(blockIdx.x, gridDim.x));
@@ -616,7 +639,7 @@ void stage_kernel(StageArgs a)
 // (scenario, tile within the scenario); scen[s] holds that scenario's
 // buffers, constants and step-control block.
 template <typename R, typename S, bool SED>
-__global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(stage_waves<R>(), stage_waves<R>())))
+__global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(stage_waves<R, SED>(), stage_waves<R, SED>())))
 void stage_kernel_batch(BatchPools pools, const BatchMember* __restrict__ mem, const int2* __restrict__ tiles, int k,
                         int last)
 {

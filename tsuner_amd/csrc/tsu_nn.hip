@@ -409,14 +409,16 @@ __global__ __launch_bounds__(256) void apply_kernel(const StageArgs* __restrict_
         store_plane<S, COMP>(V, 3, nv, p, H2);
         if (h2 == 0.0) store_plane<S, COMP>(V, 4, nv, p, 0.0);
         if (a.ext) {
+            // the stage kernels fold the stored leading words (FP32: hi)
+            const double hx = (double)V[p], Hx = (double)V[3 * nv + p];
             double* hmax = a.ext;
             double* bmax = a.ext + nv;
             double* emax = a.ext + 2 * nv;
-            if (h2 > eps_h) {
-                hmax[p] = fmax(hmax[p], h2);
-                emax[p] = fmax(emax[p], h2 - H2);
+            if (hx > eps_h) {
+                hmax[p] = fmax(hmax[p], hx);
+                emax[p] = fmax(emax[p], hx - Hx);
             }
-            bmax[p] = fmax(bmax[p], H2);
+            bmax[p] = fmax(bmax[p], Hx);
         }
     }
     if (resid) {
