@@ -57,7 +57,10 @@ def run_variant(sc, precision, steps, warmup, tile_rows):
     from tsuner_amd.solver import Solver
     bytes_per_update = 200.0 if precision == "fp64" else 100.0
     log("[bench] %s: upload %dx%d" % (precision, sc.npx, sc.npy))
+    tu = time.perf_counter()
     g = Solver.from_scenario(sc, precision=precision, tile_rows=tile_rows)
+    g.synchronize()
+    upload_s = time.perf_counter() - tu
     log("[bench] %s: warmup %d steps (tile rows %d)" % (precision, warmup, g.tile_rows))
     g.step(warmup)
     g.set_timing(True)
@@ -88,6 +91,13 @@ def run_variant(sc, precision, steps, warmup, tile_rows):
                      "algorithmic_bytes_per_launch": per_launch_bytes},
         "mass": g.mass(),
     }
+    td = time.perf_counter()
+    g.state()
+    download_s = time.perf_counter() - td
+    # host buffers in and out around the timed steps (create + upload + first
+    # dt, then one download): the rate a caller handing over host arrays sees
+    res["pcie_inclusive"] = {"upload_s": upload_s, "download_s": download_s,
+                             "cell_updates_per_s": cells * steps / (upload_s + wall + download_s)}
     log("[bench] %s: %.3f G cell-updates/s, %.3f ms/step, stage kernels %.3f/%.3f ms, %.0f GB/s"
         % (precision, res["value"] / 1e9, res["ms_per_step"], res["stage_ms_mean"][0], res["stage_ms_mean"][1],
            achieved))
@@ -235,8 +245,10 @@ def main():
         "fp32": None if r32 is None else {"value": r32["value"], "ms_per_step": r32["ms_per_step"],
                                           "dtype": "f32 state and fluxes, compensated h/H/hc (+60 B/cell-update lo planes)",
                                           "bytes_per_cell_update": 100,
-                                          "roofline": r32["roofline"]},
+                                          "roofline": r32["roofline"],
+                                          "pcie_inclusive": r32["pcie_inclusive"]},
         "stage_ms_mean": r64["stage_ms_mean"],
+        "pcie_inclusive": r64["pcie_inclusive"],
         "active_flow": active,
         "ensemble_fine": ensemble,
     }
