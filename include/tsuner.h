@@ -252,13 +252,17 @@ TSU_API int tsu_nn_pack(const float *w, void *wb, void *wt, int m, int k, int mt
  * one grid: x[member][y][x][8] BF16 = (h, qx, qy, z = -H, dz/dx, dz/dy, eta if
  * h > wet_h else 0, wet flag) times scale_in[0..7]. */
 TSU_API int tsu_ensemble_nn_features(tsu_ensemble *e, const float scale_in[8], float wet_h, void *x);
-/* Adds out[member][y][x][0..3] * scale_out to (h, qx, qy, H) of the members
- * whose step count equals expect_step; h is clipped at 0, a dry cell is only
- * wetted by more than wet_h, q and hc vanish where h = 0.  resid_max (n
- * floats on the device, or NULL) is raised to the largest |correction of
- * h - H| applied to each member (the fallback indicator). */
+/* Adds out[member][y][x][0..3] * scale_out to (h, qx, qy, H) of the wet
+ * cells (h > wet_h) of the members whose step count equals expect_step; h is
+ * clipped at 0, and q and hc vanish where it reaches 0.  Dry cells are left to
+ * the solver unless x (the network inputs of this window, as written by
+ * tsu_ensemble_nn_features) is given: then a dry cell with a wet 4-neighbour
+ * takes the depth and discharge corrections if they leave it deeper than
+ * shore_h (its bed is unchanged).  resid_max (n floats on the device, or
+ * NULL) is raised to the largest |correction of h - H| applied to a wet cell
+ * of each member (the fallback indicator). */
 TSU_API int tsu_ensemble_nn_apply(tsu_ensemble *e, const float scale_out[4], float wet_h, const float *out,
-                                  long long expect_step, float *resid_max);
+                                  long long expect_step, float *resid_max, const void *x, float shore_h);
 /* coarse member i = 4x4 block means of fine member i (fine grids 4x the
  * coarse grid in each direction, same precision), then the coarse dt. */
 TSU_API int tsu_ensemble_restrict(tsu_ensemble *fine, tsu_ensemble *coarse);

@@ -86,16 +86,41 @@ def test_features_restrict_target_apply(precision):
     coarse.synchronize()
     for i in range(2):
         B = before[i]
+        was_wet = B[0] > WET
+        assert was_wet.any() and (~was_wet).any()
         h2 = B[0] + out[i, ..., 0].astype(np.float32) * np.float32(SCALE_OUT[0])
-        h2 = np.where((h2 > 0) & ~((B[0] <= 0) & (h2 < WET)), h2, 0.0)
+        h2 = np.where(was_wet, np.maximum(h2, 0.0), B[0])
         A = coarse.state(i)
         rt = 1e-6 if precision == "fp32" else 1e-12
         np.testing.assert_allclose(A[0], h2, rtol=rt, atol=1e-9)
-        np.testing.assert_allclose(A[3], B[3] + out[i, ..., 3] * np.float32(SCALE_OUT[3]), rtol=rt, atol=1e-9)
-        wet = h2 > 0
+        H2 = np.where(was_wet, B[3] + out[i, ..., 3] * np.float32(SCALE_OUT[3]), B[3])
+        np.testing.assert_allclose(A[3], H2, rtol=rt, atol=1e-9)
+        wet = was_wet & (h2 > 0)
         np.testing.assert_allclose(A[1][wet], (B[1] + out[i, ..., 1] * np.float32(SCALE_OUT[1]))[wet], rtol=rt,
                                    atol=1e-6)
-        assert np.all(A[1][~wet] == 0) and np.all(A[2][~wet] == 0)
+        assert np.all(A[1][was_wet & ~wet] == 0) and np.all(A[2][was_wet & ~wet] == 0)
+        np.testing.assert_array_equal(A[:, ~was_wet], B[:, ~was_wet])   # dry cells are left alone
+    # shoreline mode: with the inputs given, a dry cell next to a wet one (by
+    # the inputs' wet flags) takes a depth correction that leaves it deeper
+    # than shore_h; other dry cells stay as they are
+    snap = [coarse.state(i) for i in range(2)]
+    shore_h = 0.02
+    flags = got[..., 7] != 0
+    coarse.nn_apply(SCALE_OUT, WET, tout.data_ptr(), 3, 0, x.data_ptr(), shore_h)
+    coarse.synchronize()
+    for i in range(2):
+        B, A = snap[i], coarse.state(i)
+        nb = np.zeros_like(flags[i])
+        nb[:, 1:] |= flags[i][:, :-1]
+        nb[:, :-1] |= flags[i][:, 1:]
+        nb[1:, :] |= flags[i][:-1, :]
+        nb[:-1, :] |= flags[i][1:, :]
+        h2 = B[0] + out[i, ..., 0].astype(np.float32) * np.float32(SCALE_OUT[0])
+        dry = ~(B[0] > WET)
+        takes = dry & nb & (h2 > shore_h)
+        np.testing.assert_allclose(A[0][takes], h2[takes], rtol=1e-6)
+        np.testing.assert_array_equal(A[3][takes], B[3][takes])
+        np.testing.assert_array_equal(A[:, dry & ~takes], B[:, dry & ~takes])
     # members whose step count differs are untouched
     snap = [coarse.state(i) for i in range(2)]
     coarse.nn_apply(SCALE_OUT, WET, tout.data_ptr(), 4)

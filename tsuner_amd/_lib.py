@@ -131,7 +131,7 @@ _PROTOS = {
     "tsu_nn_pack": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_int, C.c_int, C.c_int, C.c_void_p]),
     "tsu_ensemble_nn_features": (C.c_int, [C.c_void_p, C.POINTER(C.c_float), C.c_float, C.c_void_p]),
     "tsu_ensemble_nn_apply": (C.c_int, [C.c_void_p, C.POINTER(C.c_float), C.c_float, C.c_void_p, C.c_longlong,
-                                        C.c_void_p]),
+                                        C.c_void_p, C.c_void_p, C.c_float]),
     "tsu_ensemble_restrict": (C.c_int, [C.c_void_p, C.c_void_p]),
     "tsu_ensemble_nn_target": (C.c_int, [C.c_void_p, C.c_void_p, C.POINTER(C.c_float), C.c_void_p]),
     "tsu_bandwidth_probe": (C.c_int, [C.c_int, C.c_longlong, C.c_int, _dp]),

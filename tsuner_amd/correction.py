@@ -36,6 +36,8 @@ FINE_N, FACTOR, FINE_DX = 1024, 4, 10.0
 COARSE_N, COARSE_DX = FINE_N // FACTOR, FINE_DX * FACTOR
 WINDOW = 10
 WET_H = 1e-3
+# a correction may wet a dry shoreline cell only beyond this depth [m]
+SHORE_H = 1e-2
 # inputs: h, qx, qy, z, dz/dx, dz/dy, eta (wet), wet flag
 SCALE_IN = [1 / 50.0, 1 / 5.0, 1 / 5.0, 1 / 50.0, 20.0, 20.0, 2.0, 1.0]
 HERE = os.path.dirname(os.path.abspath(__file__))
@@ -146,6 +148,28 @@ def generate(seeds: Sequence[int], windows: int, scale_out: Sequence[float], pre
 SCALE_OUT_FLOOR = [1e-3, 1e-2, 1e-2, 1e-4]
 
 
+def normalise(ds: Dataset, chunk: int = 64) -> List[float]:
+    """Sets the output scales to the RMS of the raw targets over the whole
+    set (not below SCALE_OUT_FLOOR) and divides the targets by them, in
+    chunks through host memory.  Returns the scales."""
+    import torch
+    n = ds.count
+    ss = np.zeros(4)
+    cnt = 0
+    for i in range(0, n, chunk):
+        y = ds.y[i:min(n, i + chunk)].cpu().numpy().reshape(-1, 4).astype(np.float64)
+        ss += (y * y).sum(axis=0)
+        cnt += y.shape[0]
+    scale = [float(max(np.sqrt(s / max(cnt, 1)), f)) for s, f in zip(ss, SCALE_OUT_FLOOR)]
+    inv = (1.0 / np.asarray(scale, np.float64)).astype(np.float32)
+    for i in range(0, n, chunk):
+        y = ds.y[i:min(n, i + chunk)].cpu().numpy()
+        ds.y[i:min(n, i + chunk)].copy_(torch.from_numpy(np.ascontiguousarray(y * inv)))
+    torch.cuda.synchronize()
+    ds.scale_out = scale
+    return scale
+
+
 def pilot_scales(seeds: Sequence[int], windows: int, fine_n: int = FINE_N) -> List[float]:
     """RMS of the raw corrections (h, qx, qy, H) over a short pilot run, not
     below SCALE_OUT_FLOOR."""
@@ -245,9 +269,11 @@ def run_fine(scs: Sequence[S.Scenario], steps: int, precision: str = "fp32", chu
 
 
 def run_coarse(scs: Sequence[S.Scenario], t_end: np.ndarray, net: "nn.Net" = None, scale_out=None,
-               precision: str = "fp32", max_windows: int = 100000) -> RunResult:
+               precision: str = "fp32", max_windows: int = 100000, shore_h: float = None) -> RunResult:
     """Coarse ensemble to the per-member end times (clamped), with the
-    network's correction after every WINDOW steps when `net` is given."""
+    network's correction after every WINDOW steps when `net` is given (wet
+    cells only, or also dry shoreline cells deeper than shore_h after the
+    correction when shore_h is given)."""
     import torch
     from .ensemble import Ensemble
     n = len(scs)
@@ -273,7 +299,8 @@ def run_coarse(scs: Sequence[S.Scenario], t_end: np.ndarray, net: "nn.Net" = Non
                 for b0 in range(0, n, net.max_batch):
                     nb = min(net.max_batch, n - b0)
                     net.forward(x.data_ptr() + b0 * npx * npx * 16, nb, out=out.data_ptr() + b0 * npx * npx * 16)
-                ens.nn_apply(scale_out, WET_H, out.data_ptr(), WINDOW * w, resid.data_ptr())
+                ens.nn_apply(scale_out, WET_H, out.data_ptr(), WINDOW * w, resid.data_ptr(),
+                             x.data_ptr() if shore_h is not None else 0, shore_h or 0.0)
             if np.all(ens.times() >= t_end):
                 break
     wall = time.perf_counter() - t0
@@ -320,12 +347,22 @@ def metrics(fine: RunResult, coarse: RunResult, i: int, nearshore_depth: float =
     both = zone & np.isfinite(ef) & np.isfinite(ec)
     peak_f = float(np.nanmax(np.where(zone, ef, np.nan))) if np.isfinite(ef[zone]).any() else float("nan")
     peak_c = float(np.nanmax(np.where(zone, ec, np.nan))) if np.isfinite(ec[zone]).any() else float("nan")
-    rf = runup(fine.H0[i], fine.hmax[i])
+    # runup on the coarse grid for both runs: the fine run wets a coarse cell
+    # when any of its fine cells got wet (so both read the same coarse ground
+    # elevation); the fine-resolution runup is reported beside it
+    n = coarse.H0[i].shape[0]
+    fwet = (fine.hmax[i] > WET_H).reshape(n, FACTOR, n, FACTOR).any(axis=(1, 3))
+    rf = runup(coarse.H0[i], np.where(fwet, 1.0, 0.0))
     rc = runup(coarse.H0[i], coarse.hmax[i])
+    rff = runup(fine.H0[i], fine.hmax[i])
+
+    def rel(a, b):
+        return abs(a - b) / b if b > 0 else (0.0 if a == 0 else float("inf"))
+
     return {"peak_eta_fine": peak_f, "peak_eta_coarse": peak_c, "peak_eta_err": abs(peak_c - peak_f),
             "mean_abs_eta_err": float(np.mean(np.abs(ec[both] - ef[both]))) if both.any() else float("nan"),
-            "runup_fine": rf, "runup_coarse": rc,
-            "runup_rel_err": abs(rc - rf) / rf if rf > 0 else (0.0 if rc == 0 else float("inf"))}
+            "runup_fine": rf, "runup_coarse": rc, "runup_rel_err": rel(rc, rf),
+            "runup_fine_resolution": rff, "runup_rel_err_vs_fine_resolution": rel(rc, rff)}
 
 
 def save(path: str, net: "nn.Net", scale_out, info: dict):
@@ -351,10 +388,13 @@ def evaluate(net, scale_out, seeds: Sequence[int], fine_steps: int = 3000, resid
     log("[eval] fine: %.2f s wall, %.0f sims/hr" % (fine.wall_s, len(seeds) * 3600 / fine.wall_s))
     base = run_coarse(coarse_sc, fine.t_end)
     log("[eval] coarse alone: %.2f s wall" % base.wall_s)
-    corr = run_coarse(coarse_sc, fine.t_end, net=net, scale_out=scale_out)
+    corr = run_coarse(coarse_sc, fine.t_end, net=net, scale_out=scale_out, shore_h=SHORE_H)
     log("[eval] coarse + network: %.2f s wall" % corr.wall_s)
+    wet_only = run_coarse(coarse_sc, fine.t_end, net=net, scale_out=scale_out)
+    log("[eval] coarse + network, wet cells only: %.2f s wall" % wet_only.wall_s)
     mb = [metrics(fine, base, i) for i in range(len(seeds))]
     mc = [metrics(fine, corr, i) for i in range(len(seeds))]
+    mw = [metrics(fine, wet_only, i) for i in range(len(seeds))]
     flagged = corr.resid > resid_tau
     n = len(seeds)
     fine_per = fine.wall_s / n
@@ -364,6 +404,8 @@ def evaluate(net, scale_out, seeds: Sequence[int], fine_steps: int = 3000, resid
         pe = np.array([0.0 if (use_fine is not None and use_fine[i]) else m["peak_eta_err"] for i, m in enumerate(ms)])
         me = np.array([0.0 if (use_fine is not None and use_fine[i]) else m["mean_abs_eta_err"] for i, m in enumerate(ms)])
         ru = np.array([0.0 if (use_fine is not None and use_fine[i]) else m["runup_rel_err"] for i, m in enumerate(ms)])
+        rf = np.array([0.0 if (use_fine is not None and use_fine[i]) else m["runup_rel_err_vs_fine_resolution"]
+                       for i, m in enumerate(ms)])
         ok = np.isfinite(pe)
         okr = np.isfinite(ru)
 
@@ -376,6 +418,7 @@ def evaluate(net, scale_out, seeds: Sequence[int], fine_steps: int = 3000, resid
                 "nearshore_mean_abs_eta_err_m": stat(np.mean, me[np.isfinite(me)]),
                 "runup_rel_err_mean": stat(np.mean, ru[okr]),
                 "runup_rel_err_p95": stat(lambda a: np.percentile(a, 95), ru[okr]),
+                "runup_rel_err_vs_fine_resolution_mean": stat(np.mean, rf[np.isfinite(rf)]),
                 "frac_peak_eta_within_5cm": stat(np.mean, (pe[ok] <= 0.05).astype(float)),
                 "frac_runup_within_10pct": stat(np.mean, (ru[okr] <= 0.10).astype(float)),
                 "scenarios_with_runup": int(sum(m["runup_fine"] > 0 for m in ms)),
@@ -386,13 +429,22 @@ def evaluate(net, scale_out, seeds: Sequence[int], fine_steps: int = 3000, resid
         "fine": {"wall_s": fine.wall_s, "sims_per_hr": n * 3600 / fine.wall_s},
         "coarse_alone": dict(summary(mb), wall_s=base.wall_s, sims_per_hr=n * 3600 / base.wall_s),
         "coarse_nn": dict(summary(mc), wall_s=corr.wall_s, sims_per_hr=n * 3600 / corr.wall_s,
-                          speedup_vs_fine=fine.wall_s / corr.wall_s),
+                          speedup_vs_fine=fine.wall_s / corr.wall_s, shore_h_m=SHORE_H),
+        "coarse_nn_wet_cells_only": dict(summary(mw), wall_s=wet_only.wall_s,
+                                         sims_per_hr=n * 3600 / wet_only.wall_s,
+                                         speedup_vs_fine=fine.wall_s / wet_only.wall_s),
         "coarse_nn_with_fallback": dict(summary(mc, flagged), resid_tau_m=resid_tau,
                                         fallback_frac=float(flagged.mean()), wall_s=wall_fallback,
                                         sims_per_hr=n * 3600 / wall_fallback,
                                         speedup_vs_fine=fine.wall_s / wall_fallback),
         "resid_max_m": {"p50": float(np.median(corr.resid)), "p95": float(np.percentile(corr.resid, 95)),
                         "max": float(corr.resid.max())},
+        # the same run post-processed at other thresholds (cost of a fallback:
+        # the fine ensemble's wall time per member)
+        "fallback_sweep": [dict(summary(mc, corr.resid > tau), resid_tau_m=tau,
+                                fallback_frac=float((corr.resid > tau).mean()),
+                                speedup_vs_fine=fine.wall_s / (corr.wall_s + fine_per * int((corr.resid > tau).sum())))
+                           for tau in (0.05, 0.1, 0.2, 0.3, 0.5)],
     }
     return out
 
@@ -401,14 +453,14 @@ def main(argv=None):
     import argparse
     ap = argparse.ArgumentParser(description="learned coarse-grid correction: train / evaluate")
     ap.add_argument("cmd", choices=["train", "evaluate", "all"])
-    ap.add_argument("--seeds", type=int, default=28, help="training scenarios")
+    ap.add_argument("--seeds", type=int, default=48, help="training scenarios")
     ap.add_argument("--windows", type=int, default=75, help="windows per training scenario")
-    ap.add_argument("--epochs", type=int, default=30)
+    ap.add_argument("--epochs", type=int, default=60)
     ap.add_argument("--batch", type=int, default=8)
     ap.add_argument("--lr", type=float, default=2e-3)
     ap.add_argument("--heldout", type=int, default=200)
     ap.add_argument("--fine-steps", type=int, default=3000)
-    ap.add_argument("--resid-tau", type=float, default=0.5)
+    ap.add_argument("--resid-tau", type=float, default=0.2)
     ap.add_argument("--weights", default=WEIGHTS)
     ap.add_argument("--json", default=None)
     a = ap.parse_args(argv)
@@ -417,9 +469,9 @@ def main(argv=None):
     result = {}
     if a.cmd in ("train", "all"):
         t0 = time.perf_counter()
-        scale_out = pilot_scales([900001, 900002], min(20, a.windows))
+        ds = generate(list(range(1, a.seeds + 1)), a.windows, [1.0, 1.0, 1.0, 1.0])
+        scale_out = normalise(ds)
         log("[train] output scales %s" % scale_out)
-        ds = generate(list(range(1, a.seeds + 1)), a.windows, scale_out)
         t_data = time.perf_counter() - t0
         log("[train] %d pairs in %.1f s" % (ds.count, t_data))
         t1 = time.perf_counter()

@@ -645,14 +645,16 @@ TSU_API int tsu_ensemble_nn_features(tsu_ensemble* e, const float scale_in[8], f
 }
 
 TSU_API int tsu_ensemble_nn_apply(tsu_ensemble* e, const float scale_out[4], float wet_h, const float* out,
-                                  long long expect_step, float* resid_max)
+                                  long long expect_step, float* resid_max, const void* x, float shore_h)
 {
     int rc = nn_uniform(e, "tsu_ensemble_nn_apply");
     if (rc) return rc;
     if (!scale_out || !out) return tsu::set_error(TSU_ERR_ARG, "null argument");
     Guard g(e->device);
-    ECHK(tsu::nn_launch_apply(e->dargs + e->n, e->n, e->p[0].npx, e->p[0].npy, e->precision,
-                              make_scales(nullptr, scale_out, wet_h), out, expect_step, resid_max, e->stream));
+    tsu::NnScales sc = make_scales(nullptr, scale_out, wet_h);
+    sc.shore_h = shore_h;
+    ECHK(tsu::nn_launch_apply(e->dargs + e->n, e->n, e->p[0].npx, e->p[0].npy, e->precision, sc, out, expect_step,
+                              resid_max, (const uint16_t*)x, e->stream));
     return TSU_OK;
 }
 

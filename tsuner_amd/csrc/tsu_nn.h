@@ -29,7 +29,8 @@ struct NnScales {
     float in[8];
     float out[4];
     float wet_h;        // wet threshold of the eta and flag channels [m]
-    float pad[3];
+    float shore_h;      // least depth a correction may give a dry shoreline cell [m]
+    float pad[2];
 };
 
 hipError_t nn_launch_conv(int kind, const uint16_t* in, const uint16_t* w, const float* bias, const uint16_t* act,
@@ -46,10 +47,12 @@ hipError_t nn_launch_pack(const float* w, uint16_t* wb, uint16_t* wt, int m, int
 // .out is the current state); all members share npx x npy.
 hipError_t nn_launch_features(const StageArgs* args, int n, int npx, int npy, int precision, const NnScales& sc,
                               uint16_t* x, hipStream_t s);
-// Adds out * scale to members whose step count equals expect_step; if resid
-// is not null, resid[i] = max(resid[i], largest |correction of h - H|).
+// Adds out * scale to members whose step count equals expect_step (wet
+// cells; with x, the network inputs, also dry shoreline cells); if resid is
+// not null, resid[i] = max(resid[i], largest |correction of h - H|).
 hipError_t nn_launch_apply(const StageArgs* args, int n, int npx, int npy, int precision, const NnScales& sc,
-                           const float* out, long long expect_step, float* resid, hipStream_t s);
+                           const float* out, long long expect_step, float* resid, const uint16_t* x,
+                           hipStream_t s);
 // Coarse state = 4x4 block means of the fine state (fine members are 4x the
 // coarse size in both directions); also resets the coarse dt slot.
 hipError_t nn_launch_restrict(const StageArgs* fine, const StageArgs* coarse, int n, int npx, int npy,

@@ -380,7 +380,7 @@ __global__ __launch_bounds__(256) void features_kernel(const StageArgs* __restri
 template <typename S, bool COMP>
 __global__ __launch_bounds__(256) void apply_kernel(const StageArgs* __restrict__ args, int npx, int npy, NnScales sc,
                                                     const float* __restrict__ out, long long expect,
-                                                    float* __restrict__ resid)
+                                                    float* __restrict__ resid, const uint16_t* __restrict__ x)
 {
     const int i = blockIdx.y;
     const StageArgs& a = args[i];
@@ -388,16 +388,29 @@ __global__ __launch_bounds__(256) void apply_kernel(const StageArgs* __restrict_
     S* V = (S*)a.out;
     const double eps_h = a.P->epsilon_h;
     const long long nv = (long long)npx * npy;
+    // wet flag (input channel 7) of cell q before this correction
+    auto wet_in = [&](long long q) -> bool { return x[((size_t)i * nv + q) * 8 + 7] != 0; };
     float rmax = 0.f;
     for (long long p = blockIdx.x * (long long)blockDim.x + threadIdx.x; p < nv; p += (long long)gridDim.x * blockDim.x) {
         const float4 o = *(const float4*)(out + ((size_t)i * nv + p) * 4);
         const double h = plane<S, COMP>(V, 0, nv, p), H = plane<S, COMP>(V, 3, nv, p);
-        // residual indicator: largest correction of the surface (h - H)
-        rmax = fmaxf(rmax, fabsf(o.x * sc.out[0] - o.w * sc.out[3]));
         double h2 = h + (double)(o.x * sc.out[0]);
-        // a dry cell is wetted only by a correction deeper than the wet threshold
-        if (!(h2 > 0.0) || (h <= 0.0 && h2 < sc.wet_h)) h2 = 0.0;
-        const double H2 = H + (double)(o.w * sc.out[3]);
+        double H2 = H;
+        if (h > sc.wet_h) {
+            // residual indicator: largest correction of the surface (h - H)
+            rmax = fmaxf(rmax, fabsf(o.x * sc.out[0] - o.w * sc.out[3]));
+            if (!(h2 > 0.0)) h2 = 0.0;
+            H2 = H + (double)(o.w * sc.out[3]);
+        } else {
+            // a dry cell is wetted only on the shoreline (a wet neighbour in
+            // the network's inputs) and only deeper than shore_h; the bed of
+            // dry cells is left alone
+            if (!x || !(h2 > sc.shore_h)) continue;
+            const int r = (int)(p / npx), c = (int)(p % npx);
+            const bool shore = (c > 0 && wet_in(p - 1)) || (c < npx - 1 && wet_in(p + 1)) ||
+                               (r > 0 && wet_in(p - npx)) || (r < npy - 1 && wet_in(p + npx));
+            if (!shore) continue;
+        }
         double qx = 0.0, qy = 0.0;
         if (h2 > 0.0) {
             qx = (double)V[nv + p] + (double)(o.y * sc.out[1]);
@@ -500,13 +513,14 @@ hipError_t nn_launch_features(const StageArgs* args, int n, int npx, int npy, in
 }
 
 hipError_t nn_launch_apply(const StageArgs* args, int n, int npx, int npy, int precision, const NnScales& sc,
-                           const float* out, long long expect_step, float* resid, hipStream_t s)
+                           const float* out, long long expect_step, float* resid, const uint16_t* x,
+                           hipStream_t s)
 {
     const dim3 g = member_grid((long long)npx * npy, n);
     if (precision == 0)
-        apply_kernel<double, false><<<g, 256, 0, s>>>(args, npx, npy, sc, out, expect_step, resid);
+        apply_kernel<double, false><<<g, 256, 0, s>>>(args, npx, npy, sc, out, expect_step, resid, x);
     else
-        apply_kernel<float, true><<<g, 256, 0, s>>>(args, npx, npy, sc, out, expect_step, resid);
+        apply_kernel<float, true><<<g, 256, 0, s>>>(args, npx, npy, sc, out, expect_step, resid, x);
     return hipGetLastError();
 }
 

@@ -121,10 +121,12 @@ class Ensemble:
         check(lib().tsu_ensemble_nn_features(self._h, s, C.c_float(wet_h), C.c_void_p(x_addr)),
               "tsu_ensemble_nn_features")
 
-    def nn_apply(self, scale_out, wet_h: float, out_addr: int, expect_step: int, resid_addr: int = 0):
+    def nn_apply(self, scale_out, wet_h: float, out_addr: int, expect_step: int, resid_addr: int = 0,
+                 x_addr: int = 0, shore_h: float = 0.0):
         s = (C.c_float * 4)(*[float(v) for v in scale_out])
         check(lib().tsu_ensemble_nn_apply(self._h, s, C.c_float(wet_h), C.c_void_p(out_addr), int(expect_step),
-                                          C.c_void_p(resid_addr or None)), "tsu_ensemble_nn_apply")
+                                          C.c_void_p(resid_addr or None), C.c_void_p(x_addr or None),
+                                          C.c_float(shore_h)), "tsu_ensemble_nn_apply")
 
     def restrict_from(self, fine: "Ensemble"):
         check(lib().tsu_ensemble_restrict(fine._h, self._h), "tsu_ensemble_restrict")
