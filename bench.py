@@ -147,6 +147,41 @@ def run_ensemble(members, steps, seed=2024, chunk=50):
     return res
 
 
+BF16_MFMA_PEAK_TFLOPS = 2500.0
+
+
+def run_correction(heldout, fine_steps):
+    """Coarse + learned correction against the fine solver on held-out
+    Gaussian-source scenarios (1024^2 fine, 256^2 coarse), with the shipped
+    network weights; plus the network's forward throughput on the MFMA."""
+    from tsuner_amd import correction as K
+    from tsuner_amd import nn
+    if not os.path.exists(K.WEIGHTS):
+        log("[bench] correction: no trained weights at %s, leg skipped" % K.WEIGHTS)
+        return None
+    params, scale_out, info = K.load()
+    net = nn.Net(params, max_batch=32, h=K.COARSE_N, w=K.COARSE_N)
+    tn = K.time_network(net, 32)
+    log("[bench] correction network: %.3f ms per 32 images, %.1f TFLOP/s" % (tn["ms"], tn["tflops"]))
+    ev = K.evaluate(net, scale_out, list(range(300001, 300001 + heldout)), fine_steps=fine_steps)
+    keys = ("peak_eta_err_mean_m", "frac_peak_eta_within_5cm", "nearshore_mean_abs_eta_err_m", "runup_rel_err_mean",
+            "frac_runup_within_10pct", "sims_per_hr", "speedup_vs_fine", "fallback_frac")
+
+    def pick(d):
+        return {k: d[k] for k in keys if k in d}
+
+    return {"workload": "%d held-out Gaussian-source scenarios, fine 1024^2 dx=10 m (%d steps) vs coarse 256^2 "
+                        "dx=40 m to the same end times, FP32; network correction every %d coarse steps"
+                        % (heldout, fine_steps, K.WINDOW),
+            "fine_sims_per_hr": ev["fine"]["sims_per_hr"], "coarse_alone": pick(ev["coarse_alone"]),
+            "coarse_nn": pick(ev["coarse_nn"]), "coarse_nn_with_fallback": pick(ev["coarse_nn_with_fallback"]),
+            "network": {"roofline": {"bound": "mfma", "achieved": tn["tflops"], "peak": BF16_MFMA_PEAK_TFLOPS,
+                                     "unit": "TFLOP/s", "frac": tn["tflops"] / BF16_MFMA_PEAK_TFLOPS,
+                                     "traffic": None},
+                        "forward_ms_per_32_images": tn["ms"], "flop_per_pixel": tn["flop_per_pixel"],
+                        "trained_on_pairs": info.get("pairs")}}
+
+
 def cpu_baseline(sc, budget_s=12.0):
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
     import pyoracle as O
@@ -185,6 +220,8 @@ def main():
     ap.add_argument("--no-ensemble", action="store_true", help="skip the 256-member fine ensemble")
     ap.add_argument("--ensemble-members", type=int, default=256)
     ap.add_argument("--ensemble-steps", type=int, default=3000)
+    ap.add_argument("--no-correction", action="store_true", help="skip the coarse + learned-correction leg")
+    ap.add_argument("--correction-scenarios", type=int, default=64)
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
         print("bench.py measures one GPU only: this solver path runs on a single MI355X and no multi-GPU "
@@ -217,6 +254,9 @@ def main():
     ensemble = None
     if not args.no_ensemble:
         ensemble = run_ensemble(args.ensemble_members, args.ensemble_steps)
+    correction = None
+    if not args.no_correction:
+        correction = run_correction(args.correction_scenarios, args.ensemble_steps)
     cpu = None
     if not args.no_cpu:
         log("[bench] cpu baseline")
@@ -251,6 +291,7 @@ def main():
         "pcie_inclusive": r64["pcie_inclusive"],
         "active_flow": active,
         "ensemble_fine": ensemble,
+        "ensemble_coarse_nn": correction,
     }
     print(json.dumps(out), flush=True)
 

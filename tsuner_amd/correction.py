@@ -496,3 +496,24 @@ def main(argv=None):
 
 if __name__ == "__main__":
     main()
+
+
+def time_network(net: "nn.Net", n: int, reps: int = 10) -> dict:
+    """hipEvent-timed forward passes of the network over n images (inputs
+    zero, so the timing is data-independent); FLOPs counted as the MFMA
+    work of the four convolutions on the real channel counts."""
+    import torch
+    h, w = net.h, net.w
+    x = torch.zeros((n, h, w, 8), dtype=torch.bfloat16, device="cuda")
+    torch.cuda.synchronize()
+    net.forward(x, n)
+    a, b = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    a.record()
+    for _ in range(reps):
+        net.forward(x, n)
+    b.record()
+    b.synchronize()
+    ms = a.elapsed_time(b) / reps
+    flop_px = 2 * 9 * (8 * 32 + 32 * 32 + 32 * 32 + 32 * 4)
+    flops = flop_px * n * h * w
+    return {"images": n, "ms": ms, "tflops": flops / (ms * 1e-3) / 1e12, "flop_per_pixel": flop_px}

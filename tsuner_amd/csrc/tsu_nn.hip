@@ -46,7 +46,14 @@ __device__ __forceinline__ int tap_dx(int t) { return t % 3 - 1; }
 //   (multiple of 16), MOUT: channels stored per output pixel.
 //   POST 0: + bias, ReLU, BF16 out.  1: + bias, FP32 out.  2: zero where
 //   act[p][m] == 0 (ReLU derivative), BF16 out.
-// One wave computes 64 consecutive pixels of one image row; 4 waves per block.
+// A workgroup of 4 waves covers 4 consecutive rows x 64 columns of one
+// image, one row per wave.  The input rows r0-1 .. r0+4 and columns
+// c0-1 .. c0+64 are staged in LDS once (zeros outside the image: the 'same'
+// padding), so every tap's B operand is one 16-byte LDS read instead of a
+// global load re-fetched by nine taps and three waves.
+constexpr int kConvRows = 4;
+constexpr int kTileRows = kConvRows + 2, kTileCols = 64 + 2;
+
 template <int K, int M, int MOUT, int SIGN, int POST>
 __global__ __launch_bounds__(256) void conv_kernel(const uint16_t* __restrict__ in, const uint16_t* __restrict__ Wt,
                                                    const float* __restrict__ bias, const uint16_t* __restrict__ act,
@@ -55,14 +62,27 @@ __global__ __launch_bounds__(256) void conv_kernel(const uint16_t* __restrict__ 
     constexpr int KO = K / 8;                  // octets per tap
     constexpr int STEPS = (9 * KO + 3) / 4;    // MFMAs per (m-tile, n-tile)
     constexpr int MT = M / 16;
-    const int lane = threadIdx.x & 63;
-    const int segs = (W + 63) / 64;
-    const long long wave = (long long)blockIdx.x * 4 + (threadIdx.x >> 6);
-    if (wave >= (long long)N * H * segs) return;
-    const int seg = (int)(wave % segs);
-    const long long row = wave / segs;         // n * H + r
-    const int r = (int)(row % H);
+    __shared__ uint4 tile[kTileRows * kTileCols * KO];
+    const int segs = (W + 63) / 64, rblocks = (H + kConvRows - 1) / kConvRows;
+    long long blk = blockIdx.x;
+    const int seg = (int)(blk % segs);
+    blk /= segs;
+    const int r0 = (int)(blk % rblocks) * kConvRows;
+    const long long n = blk / rblocks;
+    if (n >= N) return;
     const int c0 = seg * 64;
+    for (int idx = threadIdx.x; idx < kTileRows * kTileCols * KO; idx += 256) {
+        const int q = idx % KO, rest = idx / KO;
+        const int gc = c0 - 1 + rest % kTileCols, gr = r0 - 1 + rest / kTileCols;
+        uint4 v = make_uint4(0, 0, 0, 0);
+        if (gr >= 0 && gr < H && gc >= 0 && gc < W) v = *(const uint4*)(in + (((size_t)n * H + gr) * W + gc) * K + 8 * q);
+        tile[idx] = v;
+    }
+    __syncthreads();
+    const int wv = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    const int r = r0 + wv;
+    if (r >= H) return;                        // no barrier follows
+    const long long row = n * H + r;
     const int g = lane >> 4, li = lane & 15;
 
     f32x4 acc[MT][4];
@@ -77,8 +97,6 @@ __global__ __launch_bounds__(256) void conv_kernel(const uint16_t* __restrict__ 
         const int t = o / KO, q = o % KO;
         const bool tv = t < 9;
         const int dy = SIGN * tap_dy(t), dx = SIGN * tap_dx(t);
-        const int rr = r + dy;
-        const bool rv = tv && rr >= 0 && rr < H;
         bf16x8 A[MT];
 #pragma unroll
         for (int a = 0; a < MT; a++) {
@@ -86,12 +104,11 @@ __global__ __launch_bounds__(256) void conv_kernel(const uint16_t* __restrict__ 
             if (tv) v = *(const uint4*)(Wt + ((size_t)(t * M + a * 16 + li) * K + 8 * q));
             A[a] = as_bf16x8(v);
         }
-        const uint16_t* base = in + ((size_t)(row + dy) * W) * K + 8 * q;
+        const int lrow = (wv + 1 + dy) * kTileCols + 1 + dx + li;
 #pragma unroll
         for (int b = 0; b < 4; b++) {
-            const int cc = c0 + b * 16 + li + dx;
             uint4 v = make_uint4(0, 0, 0, 0);
-            if (rv && cc >= 0 && cc < W) v = *(const uint4*)(base + (size_t)cc * K);
+            if (tv) v = tile[(lrow + b * 16) * KO + q];
             const bf16x8 B = as_bf16x8(v);
 #pragma unroll
             for (int a = 0; a < MT; a++)
@@ -549,8 +566,8 @@ hipError_t nn_launch_target(const StageArgs* fine, const StageArgs* coarse, int 
 hipError_t nn_launch_conv(int kind, const uint16_t* in, const uint16_t* w, const float* bias, const uint16_t* act,
                           void* out, int N, int H, int W, hipStream_t s)
 {
-    const long long waves = (long long)N * H * ((W + 63) / 64);
-    const dim3 grid((unsigned)((waves + 3) / 4)), block(256);
+    const long long blocks = (long long)N * ((H + kConvRows - 1) / kConvRows) * ((W + 63) / 64);
+    const dim3 grid((unsigned)blocks), block(256);
     switch (kind) {
     case NN_FWD_IN:      // 8 -> 32, ReLU
         conv_kernel<8, 32, 32, 1, 0><<<grid, block, 0, s>>>(in, w, bias, act, out, N, H, W);
