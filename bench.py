@@ -174,7 +174,8 @@ def run_correction(heldout, fine_steps):
                         "dx=40 m to the same end times, FP32; network correction every %d coarse steps"
                         % (heldout, fine_steps, K.WINDOW),
             "fine_sims_per_hr": ev["fine"]["sims_per_hr"], "coarse_alone": pick(ev["coarse_alone"]),
-            "coarse_nn": pick(ev["coarse_nn"]), "coarse_nn_with_fallback": pick(ev["coarse_nn_with_fallback"]),
+            "coarse_nn": pick(ev["coarse_nn"]),
+            "fallback_policies": {k: pick(v) for k, v in ev["fallback_policies"]["policies"].items()},
             "network": {"roofline": {"bound": "mfma", "achieved": tn["tflops"], "peak": BF16_MFMA_PEAK_TFLOPS,
                                      "unit": "TFLOP/s", "frac": tn["tflops"] / BF16_MFMA_PEAK_TFLOPS,
                                      "traffic": None},

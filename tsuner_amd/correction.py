@@ -376,7 +376,7 @@ def load(path: str = WEIGHTS):
 
 
 def evaluate(net, scale_out, seeds: Sequence[int], fine_steps: int = 3000, resid_tau: float = 0.5, fine_n: int = FINE_N,
-             max_batch: int = 32) -> dict:
+             max_batch: int = 32, min_speedup: float = 8.0) -> dict:
     """Held-out evaluation: fine members (fine_steps steps) against the coarse
     solver alone and coarse + network, with the residual fallback."""
     t0 = time.perf_counter()
@@ -446,6 +446,44 @@ def evaluate(net, scale_out, seeds: Sequence[int], fine_steps: int = 3000, resid
                                 speedup_vs_fine=fine.wall_s / (corr.wall_s + fine_per * int((corr.resid > tau).sum())))
                            for tau in (0.05, 0.1, 0.2, 0.3, 0.5)],
     }
+    # Fallback policies from run-time indicators, each threshold chosen on the
+    # first half of the scenarios (the largest fallback share that keeps
+    # >= min_speedup there) and reported on the second half.
+    disagree = np.array([abs(mc[i]["peak_eta_coarse"] - mb[i]["peak_eta_coarse"]) for i in range(n)])
+    peak_c = np.array([mc[i]["peak_eta_coarse"] for i in range(n)])
+    indicators = {"resid_max": (corr.resid, corr.wall_s),
+                  "coarse_peak_eta": (peak_c, corr.wall_s),
+                  "nn_vs_coarse_peak_disagreement": (disagree, corr.wall_s + base.wall_s)}
+    half = n // 2
+    cal, test = np.arange(half), np.arange(half, n)
+
+    def cost(ind, tau, idx, base_wall):
+        fb = ind[idx] > tau
+        return base_wall * len(idx) / n + fine_per * int(fb.sum()), fb
+
+    policies = {}
+    for name, (ind, bw) in indicators.items():
+        best = None
+        for tau in np.unique(np.concatenate([ind[cal], [np.inf]])):
+            wall, fb = cost(ind, tau, cal, bw)
+            sp = fine_per * len(cal) / wall
+            if sp >= min_speedup and (best is None or fb.mean() > best[1]):
+                best = (float(tau), float(fb.mean()))
+        tau = best[0] if best else float("inf")
+        wall, fb = cost(ind, tau, test, bw)
+        sub = [mc[i] for i in test]
+        policies[name] = dict(summary(sub, fb), threshold=tau, fallback_frac=float(fb.mean()),
+                              speedup_vs_fine=fine_per * len(test) / wall, scenarios=int(len(test)))
+    out["fallback_policies"] = {"min_speedup_on_calibration_half": min_speedup, "policies": policies}
+    out["per_scenario"] = {"seed": [int(s) for s in seeds],
+                           "peak_eta_fine": [m["peak_eta_fine"] for m in mc],
+                           "peak_eta_err_coarse": [m["peak_eta_err"] for m in mb],
+                           "peak_eta_err_nn": [m["peak_eta_err"] for m in mc],
+                           "runup_fine": [m["runup_fine"] for m in mc],
+                           "runup_rel_err_coarse": [m["runup_rel_err"] for m in mb],
+                           "runup_rel_err_nn": [m["runup_rel_err"] for m in mc],
+                           "resid_max": [float(v) for v in corr.resid],
+                           "nn_vs_coarse_peak_disagreement": [float(v) for v in disagree]}
     return out
 
 
@@ -486,6 +524,8 @@ def main(argv=None):
         params, scale_out, _ = load(a.weights)
         net = nn.Net(params, max_batch=32, h=COARSE_N, w=COARSE_N)
         seeds = list(range(100001, 100001 + a.heldout))
+        result["network_forward"] = time_network(net, 32)
+        log("[eval] network forward: %s" % result["network_forward"])
         result["evaluate"] = evaluate(net, scale_out, seeds, fine_steps=a.fine_steps, resid_tau=a.resid_tau)
     js = json.dumps(result, indent=1)
     print(js, flush=True)
