@@ -101,9 +101,14 @@ class Dataset:
 
 
 def generate(seeds: Sequence[int], windows: int, scale_out: Sequence[float], precision: str = "fp32",
-             capacity: int = None, seed_perm: int = 0, fine_n: int = FINE_N) -> Dataset:
+             capacity: int = None, seed_perm: int = 0, fine_n: int = FINE_N, net: "nn.Net" = None,
+             shore_h: float = SHORE_H) -> Dataset:
     """Training pairs from fine/coarse runs of `seeds`, `windows` windows each.
-    Pairs are written in a random order (contiguous batches are then random)."""
+    Without `net` every window starts from restrict(fine).  With `net` the
+    coarse members follow their own corrected trajectory instead (the
+    network's correction is applied after each window, as in deployment), so
+    the pairs show the network the states it meets when it runs.  Pairs are
+    written in a random order (contiguous batches are then random)."""
     import torch
     from .ensemble import Ensemble
     fine_sc = [fine_scenario(s, n=fine_n) for s in seeds]
@@ -119,11 +124,12 @@ def generate(seeds: Sequence[int], windows: int, scale_out: Sequence[float], pre
     del fine_sc, coarse_sc
     xb = torch.zeros((m, npx, npx, 8), dtype=torch.bfloat16, device="cuda")
     yb = torch.zeros((m, npx, npx, 4), dtype=torch.float32, device="cuda")
+    ob = torch.zeros((m, npx, npx, 4), dtype=torch.float32, device="cuda") if net is not None else None
     torch.cuda.synchronize()
     k = 0
     t0 = time.perf_counter()
     for w in range(windows):
-        if w > 0:
+        if w > 0 and net is None:
             coarse.restrict_from(fine)
         coarse.step(WINDOW)
         tc = coarse.times()
@@ -131,6 +137,13 @@ def generate(seeds: Sequence[int], windows: int, scale_out: Sequence[float], pre
         coarse.nn_features(SCALE_IN, WET_H, xb.data_ptr())
         coarse.nn_target(fine, scale_out, yb.data_ptr())
         coarse.synchronize()
+        if net is not None:
+            for b0 in range(0, m, net.max_batch):
+                nb = min(net.max_batch, m - b0)
+                net.forward(xb.data_ptr() + b0 * npx * npx * 16, nb, out=ob.data_ptr() + b0 * npx * npx * 16)
+            torch.cuda.synchronize()
+            coarse.nn_apply(scale_out, WET_H, ob.data_ptr(), WINDOW * (w + 1), 0, xb.data_ptr(), shore_h)
+            coarse.synchronize()
         for i in range(m):
             if k < total:
                 X[perm[k]].copy_(xb[i])
@@ -179,47 +192,47 @@ def pilot_scales(seeds: Sequence[int], windows: int, fine_n: int = FINE_N) -> Li
     return [float(max(r, f)) for r, f in zip(rms, SCALE_OUT_FLOOR)]
 
 
-def train(ds: Dataset, epochs: int, batch: int = 8, lr: float = 2e-3, seed: int = 0, net: "nn.Net" = None,
+def train(ds, epochs: int, batch: int = 8, lr: float = 2e-3, seed: int = 0, net: "nn.Net" = None,
           log_every: int = 1, val_frac: float = 0.1):
-    """Adam over contiguous batches of the (pre-shuffled) dataset, the last
-    `val_frac` of the batches held out for validation; the batch order is
-    reshuffled every epoch.  Returns the network and the per-epoch
-    validation losses (mean squared error of the normalised targets; 1.0 is
-    the loss of predicting no correction when the scales are the RMS)."""
-    n = ds.count
-    nb = n // batch
-    nval = min(int(round(nb * val_frac)), nb - 1) if nb > 1 else 0
-    ntr = nb - nval
-    h = ds.x.shape[1]
+    """Adam over contiguous batches of the (pre-shuffled) dataset or list of
+    datasets, the last `val_frac` of each one's batches held out for
+    validation; the batch order is reshuffled every epoch.  Returns the
+    network and the per-epoch validation losses (mean squared error of the
+    normalised targets; 1.0 is the loss of predicting no correction when the
+    scales are the RMS)."""
+    sets = ds if isinstance(ds, (list, tuple)) else [ds]
+    h = sets[0].x.shape[1]
+    xstride = h * h * 8 * 2
+    ystride = h * h * 4 * 4
+    train_b, val_b = [], []          # (x address, y address) per batch
+    for d in sets:
+        nb = d.count // batch
+        nval = min(int(round(nb * val_frac)), nb - 1) if nb > 1 else 0
+        for b in range(nb):
+            a = (d.x.data_ptr() + b * batch * xstride, d.y.data_ptr() + b * batch * ystride)
+            (val_b if b >= nb - nval else train_b).append(a)
     if net is None:
         net = nn.Net(nn.init_params(seed), max_batch=batch, h=h, w=h)
     rng = np.random.default_rng(seed)
     hist = []
-    base = ds.x.data_ptr(), ds.y.data_ptr()
-    xstride = h * h * 8 * 2
-    ystride = h * h * 4 * 4
-
-    def addr(b):
-        return base[0] + int(b) * batch * xstride, base[1] + int(b) * batch * ystride
 
     def val_loss():
-        if nval == 0:
+        if not val_b:
             return float("nan")
         tot = 0.0
-        for b in range(ntr, nb):
-            xa, ya = addr(b)
+        for xa, ya in val_b:
             net.forward(xa, batch)
             net.backward(xa, ya, batch)
             tot += net.loss(batch)
-        return tot / nval
+        return tot / len(val_b)
 
     t0 = time.perf_counter()
     hist.append(val_loss())
     log("[train] before training: validation loss %.4f" % hist[-1])
     for ep in range(epochs):
         lr_ep = lr * 0.5 * (1.0 + np.cos(np.pi * ep / max(1, epochs)))
-        for b in rng.permutation(ntr if ntr > 0 else nb):
-            xa, ya = addr(b)
+        for j in rng.permutation(len(train_b)):
+            xa, ya = train_b[j]
             net.forward(xa, batch)
             net.backward(xa, ya, batch)
             net.adam(lr_ep)
@@ -487,6 +500,27 @@ def evaluate(net, scale_out, seeds: Sequence[int], fine_steps: int = 3000, resid
     return out
 
 
+def time_network(net: "nn.Net", n: int, reps: int = 10) -> dict:
+    """hipEvent-timed forward passes of the network over n images (inputs
+    zero, so the timing is data-independent); FLOPs counted as the MFMA
+    work of the four convolutions on the real channel counts."""
+    import torch
+    h, w = net.h, net.w
+    x = torch.zeros((n, h, w, 8), dtype=torch.bfloat16, device="cuda")
+    torch.cuda.synchronize()
+    net.forward(x, n)
+    a, b = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    a.record()
+    for _ in range(reps):
+        net.forward(x, n)
+    b.record()
+    b.synchronize()
+    ms = a.elapsed_time(b) / reps
+    flop_px = 2 * 9 * (8 * 32 + 32 * 32 + 32 * 32 + 32 * 4)
+    flops = flop_px * n * h * w
+    return {"images": n, "ms": ms, "tflops": flops / (ms * 1e-3) / 1e12, "flop_per_pixel": flop_px}
+
+
 def main(argv=None):
     import argparse
     ap = argparse.ArgumentParser(description="learned coarse-grid correction: train / evaluate")
@@ -494,6 +528,7 @@ def main(argv=None):
     ap.add_argument("--seeds", type=int, default=48, help="training scenarios")
     ap.add_argument("--windows", type=int, default=75, help="windows per training scenario")
     ap.add_argument("--epochs", type=int, default=60)
+    ap.add_argument("--rounds", type=int, default=1, help="on-trajectory data rounds after the first training")
     ap.add_argument("--batch", type=int, default=8)
     ap.add_argument("--lr", type=float, default=2e-3)
     ap.add_argument("--heldout", type=int, default=200)
@@ -514,12 +549,25 @@ def main(argv=None):
         log("[train] %d pairs in %.1f s" % (ds.count, t_data))
         t1 = time.perf_counter()
         net, hist = train(ds, a.epochs, batch=a.batch, lr=a.lr)
+        sets, pairs, hists = [ds], ds.count, [hist]
+        # on-trajectory rounds: new scenarios run with the current network
+        # applied, so training also sees the states the network itself leads to
+        for rnd in range(a.rounds):
+            t2 = time.perf_counter()
+            seeds = list(range(1000 * (rnd + 1) + 1, 1000 * (rnd + 1) + 1 + a.seeds))
+            ds2 = generate(seeds, a.windows, scale_out, net=net, seed_perm=rnd + 1)
+            log("[train] round %d: %d on-trajectory pairs in %.1f s" % (rnd + 1, ds2.count, time.perf_counter() - t2))
+            sets.append(ds2)
+            pairs += ds2.count
+            net, hist = train(sets, max(1, a.epochs // 2), batch=a.batch, lr=a.lr / 2, net=net)
+            hists.append(hist)
         t_train = time.perf_counter() - t1
-        info = {"pairs": ds.count, "train_seeds": ds.seeds, "windows": a.windows, "epochs": a.epochs,
-                "batch": a.batch, "lr": a.lr, "loss": hist, "data_s": t_data, "train_s": t_train}
+        info = {"pairs": pairs, "train_seeds": ds.seeds, "windows": a.windows, "epochs": a.epochs,
+                "rounds": a.rounds, "batch": a.batch, "lr": a.lr, "loss": hists, "data_s": t_data,
+                "train_s": t_train}
         save(a.weights, net, scale_out, info)
         result["train"] = {k: v for k, v in info.items() if k != "train_seeds"}
-        del ds
+        del ds, sets
     if a.cmd in ("evaluate", "all"):
         params, scale_out, _ = load(a.weights)
         net = nn.Net(params, max_batch=32, h=COARSE_N, w=COARSE_N)
@@ -536,24 +584,3 @@ def main(argv=None):
 
 if __name__ == "__main__":
     main()
-
-
-def time_network(net: "nn.Net", n: int, reps: int = 10) -> dict:
-    """hipEvent-timed forward passes of the network over n images (inputs
-    zero, so the timing is data-independent); FLOPs counted as the MFMA
-    work of the four convolutions on the real channel counts."""
-    import torch
-    h, w = net.h, net.w
-    x = torch.zeros((n, h, w, 8), dtype=torch.bfloat16, device="cuda")
-    torch.cuda.synchronize()
-    net.forward(x, n)
-    a, b = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
-    a.record()
-    for _ in range(reps):
-        net.forward(x, n)
-    b.record()
-    b.synchronize()
-    ms = a.elapsed_time(b) / reps
-    flop_px = 2 * 9 * (8 * 32 + 32 * 32 + 32 * 32 + 32 * 4)
-    flops = flop_px * n * h * w
-    return {"images": n, "ms": ms, "tflops": flops / (ms * 1e-3) / 1e12, "flop_per_pixel": flop_px}
