@@ -528,7 +528,7 @@ def main(argv=None):
     ap.add_argument("--seeds", type=int, default=48, help="training scenarios")
     ap.add_argument("--windows", type=int, default=75, help="windows per training scenario")
     ap.add_argument("--epochs", type=int, default=60)
-    ap.add_argument("--rounds", type=int, default=1, help="on-trajectory data rounds after the first training")
+    ap.add_argument("--rounds", type=int, default=0, help="on-trajectory data rounds after the first training")
     ap.add_argument("--batch", type=int, default=8)
     ap.add_argument("--lr", type=float, default=2e-3)
     ap.add_argument("--heldout", type=int, default=200)
