@@ -154,6 +154,14 @@ def lib():
         if not os.path.exists(LIB_PATH):
             raise RuntimeError("libtsuner.so is not built (run __graft_entry__.build() or "
                                "python tsuner_amd/build.py); there is no CPU fallback")
+        # torch ships its own HIP runtime under the same soname
+        # (libamdhip64.so.7).  Loaded first, it serves libtsuner too; loaded
+        # after libtsuner it would bind to the system runtime and find no GPU.
+        # So when torch is installed it is imported before the library.
+        try:
+            import torch  # noqa: F401
+        except ImportError:
+            pass
         L = C.CDLL(LIB_PATH)
         for name, (res, args) in _PROTOS.items():
             fn = getattr(L, name)

@@ -35,6 +35,7 @@ namespace tsu {
 // Sim_parameters.hxx:28-36
 constexpr double kEpsilon = 2.2204460492503131e-16;  // DBL_EPSILON
 constexpr double kG = 9.81;
+constexpr double kSqrtG = 3.1320919526731650;   // sqrt(9.81), FP32 solver only
 constexpr double kEps = 1e-6;
 constexpr double kVmax = 10.0;
 // FP32 solver: cells shallower than this get their velocity in FP64.
@@ -301,10 +302,19 @@ __device__ __forceinline__ void riemann(const DevParams& P, bool bed_mobile,
 
     // Roe average (SedFlux.cuh:181-189) and HLL speeds (VolumeSolver.cu:63-73)
     R hm = half * (h0 + h1);
-    R SL = u0.x - rsqrt_(g * h0);
-    R SR = u1.x + rsqrt_(g * h1);
-    R cm = rsqrt_(g * hm);
     R sh0 = rsqrt_(h0), sh1 = rsqrt_(h1);
+    R SL, SR, cm;
+    if constexpr (sizeof(R) == 8) {
+        SL = u0.x - rsqrt_(g * h0);
+        SR = u1.x + rsqrt_(g * h1);
+        cm = rsqrt_(g * hm);
+    } else {
+        // FP32 solver: sqrt(g h) as sqrt(g) sqrt(h), reusing the Roe weights
+        const R sg = (R)kSqrtG;
+        SL = u0.x - sg * sh0;
+        SR = u1.x + sg * sh1;
+        cm = sg * rsqrt_(hm);
+    }
     R ssum = sh0 + sh1;
     if (ssum > zero) {  // both sides dry: the reference's 0/0 is dropped by fmin/fmax
         R umx = rdiv(sh0 * u0.x + sh1 * u1.x, ssum);
