@@ -24,6 +24,7 @@ from __future__ import annotations
 
 import json
 import os
+import sys
 import time
 from dataclasses import dataclass, field
 from typing import List, Sequence
@@ -46,7 +47,8 @@ WEIGHTS = os.path.join(HERE, "correction_weights.npz")
 
 
 def log(msg):
-    print(msg, flush=True)
+    # progress goes to stderr: stdout carries only results (JSON)
+    print(msg, file=sys.stderr, flush=True)
 
 
 def coarse_scenario(fine: S.Scenario) -> S.Scenario:
