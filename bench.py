@@ -55,7 +55,11 @@ def cpu_model():
 
 def run_variant(sc, precision, steps, warmup, tile_rows):
     from tsuner_amd.solver import Solver
-    bytes_per_update = 200.0 if precision == "fp64" else 100.0
+    # algorithmic bytes per cell-update of RK2: 5 evolving planes with
+    # sediment (2 reads + 1 write in stage 2, 1 + 1 in stage 1), 3 evolving
+    # planes plus the read-only bed without it
+    sed = bool(sc.scheme["bedload"] or sc.scheme["suspension"])
+    bytes_per_update = (200.0 if sed else 136.0) * (1.0 if precision == "fp64" else 0.5)
     log("[bench] %s: upload %dx%d" % (precision, sc.npx, sc.npy))
     tu = time.perf_counter()
     g = Solver.from_scenario(sc, precision=precision, tile_rows=tile_rows)
@@ -81,13 +85,15 @@ def run_variant(sc, precision, steps, warmup, tile_rows):
     achieved = per_launch_bytes / (mean_ms * 1e-3) / 1e9
     res = {
         "value": cells * steps / wall,
+        "bytes_per_cell_update": bytes_per_update,
         "ms_per_step": wall * 1e3 / steps,
         "sim_time": t, "dt": dt, "steps_done": n,
         "stage_ms_mean": [ms[0] / max(launches[0], 1), ms[1] / max(launches[1], 1)],
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS,
                      "traffic": TRAFFIC_8192[precision] if sc.npx == 8192 and sc.npy == 8192 else None,
-                     "kernel": "stage_kernel<%s, true>" % ("double, double" if precision == "fp64" else "float, float"),
+                     "kernel": "stage_kernel<%s, %s>" % ("double, double" if precision == "fp64" else "float, float",
+                                                         "true" if sed else "false"),
                      "algorithmic_bytes_per_launch": per_launch_bytes},
         "mass": g.mass(),
     }
@@ -103,6 +109,26 @@ def run_variant(sc, precision, steps, warmup, tile_rows):
            achieved))
     g.close()
     return res
+
+
+def run_small_grids(warmup):
+    """The two smaller single-grid workloads at their configured step counts:
+    512^2 plain shallow water (dx 20 m, Gaussian hump on a plane beach, no
+    sediment, walls, 2000 steps) and 2048^2 sediment shelf (dx 10 m, fractal
+    bed, non-erodible patches, 3 m / 600 s N-wave inflow, 5000 steps), FP64
+    and FP32 each."""
+    from tsuner_amd import scenarios as S
+    out = {}
+    for sc in (S.plain_swe(n=512, dx=20.0, steps=2000), S.sediment_shelf(n=2048, dx=10.0, steps=5000)):
+        row = {"cells": sc.npx * sc.npy, "steps": sc.steps}
+        for prec in ("fp64", "fp32"):
+            r = run_variant(sc, prec, sc.steps, warmup, 0)
+            row[prec] = {"value": r["value"], "ms_per_step": r["ms_per_step"],
+                         "bytes_per_cell_update": r["bytes_per_cell_update"],
+                         "roofline_achieved_gbs": r["roofline"]["achieved"],
+                         "roofline_frac": r["roofline"]["frac"], "kernel": r["roofline"]["kernel"]}
+        out[sc.name] = row
+    return out
 
 
 ENSEMBLE_SIDES = (256, 384, 512, 768, 1024)
@@ -218,6 +244,7 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-active", action="store_true",
                     help="skip the active-flow variant (every wet cell moving, sediment suspended)")
+    ap.add_argument("--no-small", action="store_true", help="skip the 512^2 and 2048^2 single-grid workloads")
     ap.add_argument("--no-ensemble", action="store_true", help="skip the 256-member fine ensemble")
     ap.add_argument("--ensemble-members", type=int, default=256)
     ap.add_argument("--ensemble-steps", type=int, default=3000)
@@ -252,6 +279,7 @@ def main():
                   "fp32_roofline_frac": None if a32 is None else a32["roofline"]["frac"],
                   "state": "initial state plus u = 1 m/s, v = 0.25 m/s and c = 1e-3 in every wet cell"}
         del sa
+    small = None if args.no_small else run_small_grids(args.warmup)
     ensemble = None
     if not args.no_ensemble:
         ensemble = run_ensemble(args.ensemble_members, args.ensemble_steps)
@@ -291,6 +319,7 @@ def main():
         "stage_ms_mean": r64["stage_ms_mean"],
         "pcie_inclusive": r64["pcie_inclusive"],
         "active_flow": active,
+        "small_grids": small,
         "ensemble_fine": ensemble,
         "ensemble_coarse_nn": correction,
     }

@@ -321,3 +321,35 @@ def test_fp32_compensated_conservation():
     say("fp32 mass drift %.3e %.3e" % ((m1[0] - m0[0]) / m0[0], (m1[1] - m0[1]) / m0[1]))
     assert abs(m1[0] - m0[0]) <= 1e-12 * m0[0]
     assert abs(m1[1] - m0[1]) <= 1e-12 * m0[1]
+
+
+def test_carrier_greenspan_fp64_parity_and_fp32_accuracy(oracle):
+    """Moving shoreline on a plane beach driven through the inflow series:
+    the FP64 path equals the oracle bit for bit over half a period, and the
+    FP32 path stays as close to the exact solution as FP64 after 1.25
+    periods (mean free-surface error and runup)."""
+    import math
+    import analytic as AN
+    A = 0.5
+    sc, B, x = AN.carrier_greenspan_scenario(4.0, A=A)
+    run_pair(sc, 1000, chunks=2)
+    t_end = 1.25 * math.pi * B.T0
+    amp = A / 4 * B.Z0
+    ru_an = AN.cg_runup(A) * B.Z0
+    res = {}
+    for prec in ("fp64", "fp32"):
+        g = Solver.from_scenario(sc, precision=prec)
+        g.set_stop_time(t_end)
+        while g.t < t_end:
+            g.step(500)
+        W = g.state()[:, 0]
+        hmax = g.extrema()[0][0]
+        e_an, _, wet_an = B.field(x, g.t, A)
+        wet = (W[0] > 1e-3) & wet_an
+        err = np.abs((W[0] - W[3]) - e_an)[wet].mean() / amp
+        ru = (-sc.V[3, 0][hmax > 1e-3]).max() / ru_an - 1.0
+        res[prec] = (err, ru)
+        say("carrier-greenspan %s: mean eta error %.2e of amplitude, runup %+.3f" % (prec, err, ru))
+        g.close()
+    assert res["fp64"][0] < 2e-3 and abs(res["fp64"][1]) < 0.03
+    assert res["fp32"][0] < 1.5 * res["fp64"][0] + 1e-4 and abs(res["fp32"][1]) < 0.03

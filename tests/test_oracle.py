@@ -359,3 +359,69 @@ def test_ritter_dam_break_convergence():
     e2, _ = dambreak_error(800)
     assert e1 < 0.02
     assert e2 < 0.6 * e1          # converges to the analytic dry-bed solution
+
+
+# ------------------------------------------------- Carrier-Greenspan runup --
+
+def test_carrier_greenspan_solution_satisfies_the_equations():
+    """The closed form itself: centred differences of (eta, u) at wet points
+    satisfy the non-dimensional shallow-water equations to truncation error,
+    and at small amplitude it reduces to the linear standing wave
+    eta = -(A/4) J0(4 sqrt(x)) sin(2t)."""
+    import analytic as AN
+    from scipy.special import j0
+    A, h = 0.5, 1e-4
+    for x0, t0 in [(0.6, 0.4), (0.05, 1.3), (1.2, 2.9), (0.3, -0.7)]:
+        E, U = {}, {}
+        for a in (-1, 0, 1):
+            e, u = AN.cg_at(x0 + a * h, np.array([t0 - h, t0, t0 + h]), A)
+            for b in (-1, 0, 1):
+                E[a, b], U[a, b] = e[b + 1], u[b + 1]
+        d = lambda a, b: x0 + a * h + E[a, b]
+        mass = (d(0, 1) - d(0, -1)) / (2 * h) + (d(1, 0) * U[1, 0] - d(-1, 0) * U[-1, 0]) / (2 * h)
+        mom = ((U[0, 1] - U[0, -1]) + U[0, 0] * (U[1, 0] - U[-1, 0]) + (E[1, 0] - E[-1, 0])) / (2 * h)
+        assert abs(mass) < 1e-6 and abs(mom) < 1e-6, (x0, t0, mass, mom)
+    # the field solver (sigma grid) agrees with the point solver
+    xs = np.array([0.2, 0.7, 1.1])
+    e1, u1, wet = AN.cg_field(xs, 0.9, A)
+    e2, u2 = np.array([AN.cg_at(x, np.array([0.9]), A) for x in xs])[:, :, 0].T
+    assert wet.all()
+    np.testing.assert_allclose(e1, e2, atol=1e-7)
+    np.testing.assert_allclose(u1, u2, atol=1e-7)
+    # linear limit
+    a = 1e-4
+    for x0 in (0.1, 0.5, 1.0):
+        t = np.array([0.3, 1.1])
+        e, _ = AN.cg_at(x0, t, a)
+        np.testing.assert_allclose(e, -a / 4 * j0(4 * np.sqrt(x0)) * np.sin(2 * t), rtol=2e-3, atol=1e-12)
+    # shoreline: sigma = 0 gives u = (A/2) cos(lambda), eta = -(A/4) sin(lambda) - u^2/2
+    assert AN.cg_runup(A) == pytest.approx(A / 4, rel=1e-9)   # max at sin(lambda) = -1
+
+
+def _cg_run(dx, periods=1.25):
+    import analytic as AN
+    import pyoracle
+    A = 0.5
+    sc, B, x = AN.carrier_greenspan_scenario(dx, A=A)
+    o = pyoracle.from_scenario(sc)
+    t_end = periods * math.pi * B.T0          # the standing wave's period is pi T0
+    while o.t < t_end:
+        o.step(1)
+    W = o.state()[:, 0]
+    hmax = o.extrema()[0][0]
+    e_an, _, wet_an = B.field(x, o.t, A)
+    wet = (W[0] > 1e-3) & wet_an
+    amp = A / 4 * B.Z0
+    eta_err = np.abs((W[0] - W[3]) - e_an)[wet].mean() / amp
+    runup = (-sc.V[3, 0][hmax > 1e-3]).max()          # highest wetted ground
+    return eta_err, runup / (AN.cg_runup(A) * B.Z0) - 1.0
+
+
+def test_carrier_greenspan_runup_convergence():
+    """Moving-shoreline accuracy on a 1:20 plane beach (6.25 m runup): the
+    free surface after 1.25 periods and the runup converge to the exact
+    nonlinear solution as dx halves."""
+    e8, r8 = _cg_run(8.0)
+    e4, r4 = _cg_run(4.0)
+    assert e8 < 5e-3 and e4 < 0.6 * e8, (e8, e4)
+    assert abs(r4) < 0.03 and abs(r4) < abs(r8), (r8, r4)
