@@ -263,8 +263,11 @@ TSU_API int tsu_ensemble_nn_features(tsu_ensemble *e, const float scale_in[8], f
  * of each member (the fallback indicator). */
 TSU_API int tsu_ensemble_nn_apply(tsu_ensemble *e, const float scale_out[4], float wet_h, const float *out,
                                   long long expect_step, float *resid_max, const void *x, float shore_h);
-/* coarse member i = 4x4 block means of fine member i (fine grids 4x the
- * coarse grid in each direction, same precision), then the coarse dt. */
+/* coarse member i = free-surface restriction of fine member i (fine grids 4x
+ * the coarse grid in each direction, same precision), then the coarse dt.
+ * Per 4x4 block: H = block mean; h = max(mean of h - H over fine cells with
+ * h > 1e-3 m, plus that H, 0), 0 if none; qx, qy, hc = block totals times
+ * h / (block total of h). */
 TSU_API int tsu_ensemble_restrict(tsu_ensemble *fine, tsu_ensemble *coarse);
 /* target[i][y][x][0..3] = (restrict(fine_i) - coarse_i) / scale_out for
  * (h, qx, qy, H), FP32. */

@@ -7,15 +7,44 @@ from tsuner_amd import correction as K
 from tsuner_amd import scenarios as S
 
 
-def test_coarse_scenario_is_block_mean():
+def test_coarse_scenario_restricts_the_free_surface():
     f = S.gaussian_source(32, 10.0, seed=4)
     c = K.coarse_scenario(f)
     assert (c.npx, c.npy) == (8, 8)
     assert c.params["dx"] == 40.0 and c.params["nvol"] == 64
-    np.testing.assert_allclose(c.V[0, 2, 3], f.V[0, 8:12, 12:16].mean(), rtol=1e-15)
-    np.testing.assert_allclose(c.V.sum(axis=(1, 2)) * 16, f.V.sum(axis=(1, 2)), rtol=1e-12)
+    # bed: block means; fully wet blocks: block means of h as well
+    np.testing.assert_allclose(c.V[3], f.V[3].reshape(8, 4, 8, 4).mean(axis=(1, 3)), rtol=1e-14)
+    wet = (f.V[0] > K.RESTRICT_WET_H).reshape(8, 4, 8, 4).sum(axis=(1, 3))
+    full = wet == 16
+    assert full.any()
+    np.testing.assert_allclose(c.V[0][full], f.V[0].reshape(8, 4, 8, 4).mean(axis=(1, 3))[full], rtol=1e-12)
     blk = f.fixH[8:12, 12:16]
     assert c.fixH[2, 3] == blk.min()
+
+
+def test_restriction_keeps_a_lake_at_rest_across_the_shoreline():
+    """A 4x4 block half sea (1 m deep) and half land (1 m high): a block mean
+    of h would give a 0.5 m surface above datum; the restriction keeps the
+    surface at datum (dry coarse cell, bed at datum), and wet blocks keep
+    their velocity and concentration."""
+    V = np.zeros((5, 8, 8))
+    V[3] = 1.0                      # depth below datum
+    V[3][:, :2] = -1.0              # land in the two left columns
+    V[0] = np.maximum(V[3], 0.0)    # still water at datum
+    R = K.restrict_state(V)
+    assert R[3, 0, 0] == 0.0 and R[0, 0, 0] == 0.0           # straddling block: dry, bed at datum
+    assert np.all(R[0, :, 1] == 1.0) and np.all(R[3, :, 1] == 1.0)
+    assert np.all(R[0] - R[3] <= 0.0 + 1e-15)                # surface never above datum
+    V[1] = 0.3 * V[0]
+    V[4] = 0.01 * V[0]
+    R = K.restrict_state(V)
+    np.testing.assert_allclose(R[1, :, 1], 0.3)
+    np.testing.assert_allclose(R[4, :, 1], 0.01)
+    # a wave crest of 0.2 m on the wet half raises the straddling block's
+    # surface to 0.2 m over a bed at datum
+    V[0][:, 2:4] += 0.2
+    R = K.restrict_state(V)
+    np.testing.assert_allclose(R[0, :, 0], 0.2)
 
 
 def test_block_mean_ignores_never_wet_cells():
@@ -48,3 +77,24 @@ def test_metrics_zero_for_identical_runs():
     m = K.metrics(fine, coarse, 0)
     assert m["peak_eta_err"] == 0.0 and m["mean_abs_eta_err"] == 0.0
     assert m["runup_fine"] > 0
+
+
+def test_subgrid_runup_reads_the_fine_ground():
+    """A coarse cell reached by water floods the fine land cells of its block
+    lying more than WET_H below its highest surface; the runup is the highest
+    of them, so a coarse run whose surface matches the fine run's gives the
+    fine run's runup, not the coarse cells' mean ground."""
+    n = 4
+    z = np.tile(np.linspace(-3.0, 3.0, 4 * n), (4 * n, 1))        # ground rising to the right
+    H0 = -z
+    eta = 1.3                                                      # still surface reached 1.3 m
+    hmax_f = np.where(z < eta, eta - z, 0.0)
+    rf = K.runup(H0, hmax_f)
+    assert rf == z[0][z[0] < eta].max()
+    Hc = H0.reshape(n, 4, n, 4).mean(axis=(1, 3))
+    hc = np.maximum(eta + Hc, 0.0)                                 # coarse depth for the same surface
+    ec = np.where(hc > K.WET_H, eta, np.nan)
+    assert K.runup_subgrid(H0, ec, hc) == rf
+    assert K.runup(Hc, hc) != rf                                   # block-mean ground differs
+    # a coarse cell never reached floods nothing
+    assert K.runup_subgrid(H0, np.full((n, n), np.nan), np.zeros((n, n))) == 0.0

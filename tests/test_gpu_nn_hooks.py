@@ -1,5 +1,5 @@
 """Solver-side hooks of the learned correction: network inputs from member
-states, 4x4 restriction, training targets, applying a correction, per-member
+states, 4x4 free-surface restriction, training targets, applying a correction, per-member
 extrema, and stop times with the crossing step clamped.  Checked against
 numpy restatements of the documented formulas and against the single-grid
 solver."""
@@ -7,6 +7,7 @@ import numpy as np
 import pytest
 
 import nn_oracle as O
+from tsuner_amd import correction as K
 from tsuner_amd import scenarios as S
 
 pytestmark = pytest.mark.gpu
@@ -49,8 +50,11 @@ def test_features_restrict_target_apply(precision):
     coarse.restrict_from(fine)
     for i in range(2):
         F = fine.state(i)
-        Rf = F.reshape(5, 16, 4, 16, 4).mean(axis=(2, 4))
-        np.testing.assert_allclose(coarse.state(i), Rf, rtol=1e-6 if precision == "fp32" else 1e-14, atol=1e-12)
+        Rf = K.restrict_state(F)
+        # blocks across the shoreline exercise the free-surface rule
+        wet = (F[0] > K.RESTRICT_WET_H).reshape(16, 4, 16, 4).sum(axis=(1, 3))
+        assert ((wet > 0) & (wet < 16)).any()
+        np.testing.assert_allclose(coarse.state(i), Rf, rtol=1e-6 if precision == "fp32" else 1e-13, atol=1e-12)
     coarse.step(3)
     # inputs
     # the hooks run on the ensemble's own stream: finish torch's fills first
@@ -71,7 +75,7 @@ def test_features_restrict_target_apply(precision):
     t = tgt.cpu().numpy().reshape(2, 16, 16, 4)
     for i in range(2):
         F, Cs = fine.state(i), coarse.state(i)
-        Rf = F.reshape(5, 16, 4, 16, 4).mean(axis=(2, 4))
+        Rf = K.restrict_state(F)
         for j in range(4):
             np.testing.assert_allclose(t[i, ..., j], ((Rf[j] - Cs[j]) / SCALE_OUT[j]).astype(np.float32),
                                        rtol=1e-5, atol=1e-4)

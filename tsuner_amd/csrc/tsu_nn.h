@@ -22,6 +22,10 @@ enum NnKind {
 // Waves of the weight-gradient contraction (partial sums per wave).
 constexpr int kWgradWaves = 256;
 
+// Fine cells deeper than this count as wet when a 4x4 block is restricted
+// onto a coarse cell [m].
+constexpr double kRestrictWetH = 1e-3;
+
 // Feature and correction scales (normalisation of the network's inputs and
 // outputs).  Input channels: h, qx, qy, z = -H, dz/dx, dz/dy, eta (wet
 // cells, else 0), wet flag.  Outputs: corrections of h, qx, qy, H.

@@ -458,6 +458,43 @@ __global__ __launch_bounds__(256) void apply_kernel(const StageArgs* __restrict_
     }
 }
 
+// Free-surface restriction of the 4x4 block of fine cells under coarse cell
+// (r, c): bed = block mean of H; surface = mean of h - H over the block's wet
+// fine cells (h > kRestrictWetH), depth = max(surface + bed, 0) (0 when no
+// fine cell is wet); discharge and suspended load = the block totals of q and
+// hc scaled by depth / block total of h (mass-weighted velocity and
+// concentration times the new depth).  A lake at rest restricts to a lake at
+// rest: a block mean of h would lift the surface of every block that
+// straddles the shoreline.
+template <typename S, bool COMP>
+__device__ __forceinline__ void restrict_block(const S* F, long long fnv, int fnx, int r, int c, double v[5])
+{
+    double sh = 0.0, sqx = 0.0, sqy = 0.0, sH = 0.0, shc = 0.0, seta = 0.0;
+    int nw = 0;
+    for (int dr = 0; dr < 4; dr++)
+        for (int dc = 0; dc < 4; dc++) {
+            const long long q = (long long)(4 * r + dr) * fnx + 4 * c + dc;
+            const double h = plane<S, COMP>(F, 0, fnv, q), H = plane<S, COMP>(F, 3, fnv, q);
+            sh += h;
+            sqx += plane<S, COMP>(F, 1, fnv, q);
+            sqy += plane<S, COMP>(F, 2, fnv, q);
+            sH += H;
+            shc += plane<S, COMP>(F, 4, fnv, q);
+            if (h > kRestrictWetH) {
+                seta += h - H;
+                nw++;
+            }
+        }
+    const double Hc = sH * 0.0625;
+    const double hc = nw ? fmax(seta / (double)nw + Hc, 0.0) : 0.0;
+    const double k = (hc > 0.0 && sh > 0.0) ? hc / sh : 0.0;
+    v[0] = hc;
+    v[1] = sqx * k;
+    v[2] = sqy * k;
+    v[3] = Hc;
+    v[4] = shc * k;
+}
+
 template <typename S, bool COMP>
 __global__ __launch_bounds__(256) void restrict_kernel(const StageArgs* __restrict__ fine,
                                                        const StageArgs* __restrict__ coarse, int npx, int npy)
@@ -468,14 +505,9 @@ __global__ __launch_bounds__(256) void restrict_kernel(const StageArgs* __restri
     const long long nv = (long long)npx * npy, fnv = 16 * nv;
     const int fnx = 4 * npx;
     for (long long p = blockIdx.x * (long long)blockDim.x + threadIdx.x; p < nv; p += (long long)gridDim.x * blockDim.x) {
-        const int r = (int)(p / npx), c = (int)(p % npx);
-        for (int f = 0; f < 5; f++) {
-            double s = 0.0;
-            for (int dr = 0; dr < 4; dr++)
-                for (int dc = 0; dc < 4; dc++)
-                    s += plane<S, COMP>(F, f, fnv, (long long)(4 * r + dr) * fnx + 4 * c + dc);
-            store_plane<S, COMP>(Cs, f, nv, p, s * 0.0625);
-        }
+        double v[5];
+        restrict_block<S, COMP>(F, fnv, fnx, (int)(p / npx), (int)(p % npx), v);
+        for (int f = 0; f < 5; f++) store_plane<S, COMP>(Cs, f, nv, p, v[f]);
     }
 }
 
@@ -489,18 +521,11 @@ __global__ __launch_bounds__(256) void target_kernel(const StageArgs* __restrict
     const S* Cs = (const S*)coarse[i].out;
     const long long nv = (long long)npx * npy, fnv = 16 * nv;
     const int fnx = 4 * npx;
-    const int fields[4] = {0, 1, 2, 3};
     for (long long p = blockIdx.x * (long long)blockDim.x + threadIdx.x; p < nv; p += (long long)gridDim.x * blockDim.x) {
-        const int r = (int)(p / npx), c = (int)(p % npx);
+        double v[5];
+        restrict_block<S, COMP>(F, fnv, fnx, (int)(p / npx), (int)(p % npx), v);
         float o[4];
-        for (int j = 0; j < 4; j++) {
-            const int f = fields[j];
-            double s = 0.0;
-            for (int dr = 0; dr < 4; dr++)
-                for (int dc = 0; dc < 4; dc++)
-                    s += plane<S, COMP>(F, f, fnv, (long long)(4 * r + dr) * fnx + 4 * c + dc);
-            o[j] = (float)((s * 0.0625 - plane<S, COMP>(Cs, f, nv, p)) / (double)sc.out[j]);
-        }
+        for (int j = 0; j < 4; j++) o[j] = (float)((v[j] - plane<S, COMP>(Cs, j, nv, p)) / (double)sc.out[j]);
         *(float4*)(target + ((size_t)i * nv + p) * 4) = make_float4(o[0], o[1], o[2], o[3]);
     }
 }
