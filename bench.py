@@ -181,12 +181,10 @@ def run_correction(heldout, fine_steps):
     Gaussian-source scenarios (1024^2 fine, 256^2 coarse), with the shipped
     network weights; plus the network's forward throughput on the MFMA."""
     from tsuner_amd import correction as K
-    from tsuner_amd import nn
     if not os.path.exists(K.WEIGHTS):
         log("[bench] correction: no trained weights at %s, leg skipped" % K.WEIGHTS)
         return None
-    params, scale_out, info = K.load()
-    net = nn.Net(params, max_batch=32, h=K.COARSE_N, w=K.COARSE_N)
+    net, scale_out, info = K.load_net()
     tn = K.time_network(net, 32)
     log("[bench] correction network: %.3f ms per 32 images, %.1f TFLOP/s" % (tn["ms"], tn["tflops"]))
     ev = K.evaluate(net, scale_out, list(range(300001, 300001 + heldout)), fine_steps=fine_steps)
@@ -206,7 +204,7 @@ def run_correction(heldout, fine_steps):
                                      "unit": "TFLOP/s", "frac": tn["tflops"] / BF16_MFMA_PEAK_TFLOPS,
                                      "traffic": None},
                         "forward_ms_per_32_images": tn["ms"], "flop_per_pixel": tn["flop_per_pixel"],
-                        "trained_on_pairs": info.get("pairs")}}
+                        "trained_on_pairs": info.get("pairs"), "arch": net.arch.as_dict()}}
 
 
 def cpu_baseline(sc, budget_s=12.0):

@@ -226,14 +226,15 @@ enum {
     TSU_NN_DGRAD_MID = 4,  /* dy[32] -> dx[32] * (act > 0); w: [9][32][32]      */
     TSU_NN_WGRAD_IN = 5,   /* dW[9][32][8], db[32] from x[8], dy[32]            */
     TSU_NN_WGRAD_MID = 6,  /* dW[9][32][32], db[32] from x[32], dy[32]          */
-    TSU_NN_WGRAD_OUT = 7   /* dW[9][mrows][32] rows 0..7, db[8] from x[32], dy[8] */
+    TSU_NN_WGRAD_OUT = 7,  /* dW[9][mrows][32] rows 0..7, db[8] from x[32], dy[8] */
+    TSU_NN_W64 = 16        /* OR-ed with a kind: hidden width 64 instead of 32  */
 };
-/* Convolution kinds 0-4.  act: the layer input's activations (ReLU mask of
- * the data-gradient kinds), else unused. */
+/* Convolution kinds 0-4 (optionally | TSU_NN_W64).  act: the layer input's
+ * activations (ReLU mask of the data-gradient kinds), else unused. */
 TSU_API int tsu_nn_conv(int kind, const void *in, const void *w, const float *bias, const void *act, void *out,
                         int n, int h, int w_, void *stream);
-/* Weight-gradient kinds 5-7 (FP32 dW, db); workspace: FP32 scratch of
- * tsu_nn_wgrad_workspace(kind) floats. */
+/* Weight-gradient kinds 5-7 (optionally | TSU_NN_W64; FP32 dW, db);
+ * workspace: FP32 scratch of tsu_nn_wgrad_workspace(kind) floats. */
 TSU_API long long tsu_nn_wgrad_workspace(int kind);
 TSU_API int tsu_nn_wgrad(int kind, const void *x, const void *dy, float *dw, float *db, int mrows, int n, int h,
                          int w_, float *workspace, long long workspace_floats, void *stream);

@@ -73,11 +73,16 @@ def rand_bf16(rng, shape, scale=1.0):
     return O.bf16(rng.normal(0, scale, size=shape).astype(np.float32))
 
 
-def test_mfma_conv_forward_kinds():
+W64 = 16   # TSU_NN_W64
+
+
+@pytest.mark.parametrize("width", [32, 64])
+def test_mfma_conv_forward_kinds(width):
     rng = np.random.default_rng(1)
     n, h, w = SHAPE
     s = C.c_void_p(0)
-    for kind, k, m, mreal in [(0, 8, 32, 32), (1, 32, 32, 32), (2, 32, 16, 4)]:
+    f = W64 if width == 64 else 0
+    for kind, k, m, mreal in [(0, 8, width, width), (1, width, width, width), (2, width, 16, 4)]:
         x = rand_bf16(rng, (n, h, w, k))
         wt = rand_bf16(rng, (9, m, k), 0.2)
         if mreal < m:
@@ -88,7 +93,7 @@ def test_mfma_conv_forward_kinds():
             out = torch.zeros(n * h * w * 4, dtype=torch.float32, device="cuda")
         else:
             out = torch.zeros(n * h * w * m, dtype=torch.bfloat16, device="cuda")
-        ok(lib().tsu_nn_conv(kind, P(dev_bf16(x)), P(dev_bf16(wt)), P(dev_f32(b)), None, P(out), n, h, w, s))
+        ok(lib().tsu_nn_conv(kind | f, P(dev_bf16(x)), P(dev_bf16(wt)), P(dev_f32(b)), None, P(out), n, h, w, s))
         got = host(out).reshape(n, h, w, -1)
         if kind == 2:
             np.testing.assert_allclose(got, ref, rtol=0, atol=1e-5 * np.abs(ref).max())
@@ -96,40 +101,44 @@ def test_mfma_conv_forward_kinds():
             assert_bf16_close(got, O.bf16(np.maximum(ref, 0).astype(np.float32)))
 
 
-def test_mfma_conv_data_gradients_with_relu_mask():
+@pytest.mark.parametrize("width", [32, 64])
+def test_mfma_conv_data_gradients_with_relu_mask(width):
     rng = np.random.default_rng(2)
     n, h, w = SHAPE
     s = C.c_void_p(0)
-    # through the output layer: dy has 8 channels (4 used), weights [9][16][32]
-    w4 = rand_bf16(rng, (9, 16, 32), 0.2)
+    f, W = (W64 if width == 64 else 0), width
+    # through the output layer: dy has 8 channels (4 used), weights [9][16][W]
+    w4 = rand_bf16(rng, (9, 16, W), 0.2)
     w4[:, 4:, :] = 0
     dy = np.zeros((n, h, w, 8), np.float32)
     dy[..., :4] = rand_bf16(rng, (n, h, w, 4))
-    act = np.maximum(rand_bf16(rng, (n, h, w, 32)), 0)
-    wb = torch.zeros(9 * 16 * 32, dtype=torch.bfloat16, device="cuda")
-    wtr = torch.zeros(9 * 32 * 8, dtype=torch.bfloat16, device="cuda")
-    ok(lib().tsu_nn_pack(P(dev_f32(w4)), P(wb), P(wtr), 16, 32, 8, s))
-    np.testing.assert_array_equal(host(wb).reshape(9, 16, 32), w4)
-    np.testing.assert_array_equal(host(wtr).reshape(9, 32, 8), np.transpose(w4[:, :8, :], (0, 2, 1)))
-    out = torch.zeros(n * h * w * 32, dtype=torch.bfloat16, device="cuda")
-    ok(lib().tsu_nn_conv(3, P(dev_bf16(dy)), P(wtr), None, P(dev_bf16(act)), P(out), n, h, w, s))
+    act = np.maximum(rand_bf16(rng, (n, h, w, W)), 0)
+    wb = torch.zeros(9 * 16 * W, dtype=torch.bfloat16, device="cuda")
+    wtr = torch.zeros(9 * W * 8, dtype=torch.bfloat16, device="cuda")
+    ok(lib().tsu_nn_pack(P(dev_f32(w4)), P(wb), P(wtr), 16, W, 8, s))
+    np.testing.assert_array_equal(host(wb).reshape(9, 16, W), w4)
+    np.testing.assert_array_equal(host(wtr).reshape(9, W, 8), np.transpose(w4[:, :8, :], (0, 2, 1)))
+    out = torch.zeros(n * h * w * W, dtype=torch.bfloat16, device="cuda")
+    ok(lib().tsu_nn_conv(3 | f, P(dev_bf16(dy)), P(wtr), None, P(dev_bf16(act)), P(out), n, h, w, s))
     ref = O.conv_input_grad(dy[..., :4], w4[:, :4, :]) * (act > 0)
-    assert_bf16_close(host(out).reshape(n, h, w, 32), O.bf16(ref.astype(np.float32)))
+    assert_bf16_close(host(out).reshape(n, h, w, W), O.bf16(ref.astype(np.float32)))
     # middle layer
-    w2 = rand_bf16(rng, (9, 32, 32), 0.2)
-    d = rand_bf16(rng, (n, h, w, 32))
-    wt2 = torch.zeros(9 * 32 * 32, dtype=torch.bfloat16, device="cuda")
-    ok(lib().tsu_nn_pack(P(dev_f32(w2)), P(torch.zeros_like(wt2)), P(wt2), 32, 32, 32, s))
-    ok(lib().tsu_nn_conv(4, P(dev_bf16(d)), P(wt2), None, P(dev_bf16(act)), P(out), n, h, w, s))
+    w2 = rand_bf16(rng, (9, W, W), 0.2)
+    d = rand_bf16(rng, (n, h, w, W))
+    wt2 = torch.zeros(9 * W * W, dtype=torch.bfloat16, device="cuda")
+    ok(lib().tsu_nn_pack(P(dev_f32(w2)), P(torch.zeros_like(wt2)), P(wt2), W, W, W, s))
+    ok(lib().tsu_nn_conv(4 | f, P(dev_bf16(d)), P(wt2), None, P(dev_bf16(act)), P(out), n, h, w, s))
     ref = O.conv_input_grad(d, w2) * (act > 0)
-    assert_bf16_close(host(out).reshape(n, h, w, 32), O.bf16(ref.astype(np.float32)))
+    assert_bf16_close(host(out).reshape(n, h, w, W), O.bf16(ref.astype(np.float32)))
 
 
-def test_mfma_weight_gradients():
+@pytest.mark.parametrize("width", [32, 64])
+def test_mfma_weight_gradients(width):
     rng = np.random.default_rng(3)
     n, h, w = SHAPE
     s = C.c_void_p(0)
-    for kind, kx, my, mrows in [(5, 8, 32, 32), (6, 32, 32, 32), (7, 32, 8, 16)]:
+    f, W = (W64 if width == 64 else 0), width
+    for kind, kx, my, mrows in [(5 | f, 8, W, W), (6 | f, W, W, W), (7 | f, W, 8, 16)]:
         x = rand_bf16(rng, (n, h, w, kx))
         dy = rand_bf16(rng, (n, h, w, my))
         dw = torch.full((9 * mrows * kx,), 7.0, dtype=torch.float32, device="cuda")
@@ -174,7 +183,8 @@ def test_mse_and_adam_kernels():
                                atol=1e-7)
 
 
-def test_network_gradients_match_oracle():
+@pytest.mark.parametrize("width,mids", [(32, 2), (64, 1), (32, 4)])
+def test_network_gradients_match_oracle(width, mids):
     """Full forward + backward on the GPU against the oracle's loss_and_grads
     (pinned by finite differences in test_nn_oracle.py), run with BF16
     rounding at the points where the GPU stores BF16 (hidden activations, the
@@ -182,17 +192,19 @@ def test_network_gradients_match_oracle():
     rest being FP32 accumulation and the rare BF16 rounding flips it causes.
     Against the unrounded FP64 oracle the same gradients agree to 8%."""
     from tsuner_amd import nn
+    arch = nn.Arch(width, mids)
+    last = arch.nlayers - 1
     rng = np.random.default_rng(5)
     n, h, w = 2, 12, 20
-    p = nn.init_params(7)
+    p = nn.init_params(7, arch)
     # give the output layer full-size weights so every layer carries gradient
-    wo, bo = nn.OFFS[3]
-    p[wo:wo + 9 * 16 * 32] *= 10.0
-    for li, (m, k) in enumerate(nn.LAYERS):
-        p[nn.OFFS[li][1]:nn.OFFS[li][1] + m] = rng.normal(0, 0.05, m)
+    wo, bo = arch.offs[last]
+    p[wo:wo + 9 * 16 * width] *= 10.0
+    for li, (m, k) in enumerate(arch.layers):
+        p[arch.offs[li][1]:arch.offs[li][1] + m] = rng.normal(0, 0.05, m)
     p[bo + 4:bo + 16] = 0
     p = O.bf16(p)
-    net = nn.Net(p, max_batch=n, h=h, w=w)
+    net = nn.Net(p, max_batch=n, h=h, w=w, arch=arch)
     x = rand_bf16(rng, (n, h, w, 8))
     tgt = rng.normal(0, 1, size=(n, h, w, 4)).astype(np.float32)
     tx, tt = dev_bf16(x), dev_f32(tgt)
@@ -201,21 +213,21 @@ def test_network_gradients_match_oracle():
     gpu_loss = net.loss(n)
     g = net.g.cpu().numpy()
     params = []
-    for li, (m, k) in enumerate(nn.LAYERS):
-        wo, bo = nn.OFFS[li]
+    for li, (m, k) in enumerate(arch.layers):
+        wo, bo = arch.offs[li]
         W = p[wo:wo + 9 * m * k].reshape(9, m, k).astype(np.float64)
         b = p[bo:bo + m].astype(np.float64)
-        if li == 3:
+        if li == last:
             W, b = W[:, :4, :], b[:4]
         params.append((W, b))
     loss, grads = O.loss_and_grads(x.astype(np.float64), tgt.astype(np.float64), params, rnd=O.bf16_64)
     loss64, grads64 = O.loss_and_grads(x.astype(np.float64), tgt.astype(np.float64), params)
     assert abs(gpu_loss - loss) <= 1e-3 * loss
-    for li, (m, k) in enumerate(nn.LAYERS):
-        wo, bo = nn.OFFS[li]
+    for li, (m, k) in enumerate(arch.layers):
+        wo, bo = arch.offs[li]
         gw = g[wo:wo + 9 * m * k].reshape(9, m, k)
         gb = g[bo:bo + m]
-        if li == 3:
+        if li == last:
             assert np.all(gw[:, 4:, :] == 0) and np.all(gb[4:] == 0)
             gw, gb = gw[:, :4, :], gb[:4]
         for (rw, rb), tol in ((grads[li], 0.01), (grads64[li], 0.08)):

@@ -227,7 +227,7 @@ def pilot_scales(seeds: Sequence[int], windows: int, fine_n: int = FINE_N) -> Li
 
 
 def train(ds, epochs: int, batch: int = 8, lr: float = 2e-3, seed: int = 0, net: "nn.Net" = None,
-          log_every: int = 1, val_frac: float = 0.1):
+          log_every: int = 1, val_frac: float = 0.1, arch: "nn.Arch" = None):
     """Adam over contiguous batches of the (pre-shuffled) dataset or list of
     datasets, the last `val_frac` of each one's batches held out for
     validation; the batch order is reshuffled every epoch.  Returns the
@@ -246,7 +246,8 @@ def train(ds, epochs: int, batch: int = 8, lr: float = 2e-3, seed: int = 0, net:
             a = (d.x.data_ptr() + b * batch * xstride, d.y.data_ptr() + b * batch * ystride)
             (val_b if b >= nb - nval else train_b).append(a)
     if net is None:
-        net = nn.Net(nn.init_params(seed), max_batch=batch, h=h, w=h)
+        arch = arch or nn.DEFAULT
+        net = nn.Net(nn.init_params(seed, arch), max_batch=batch, h=h, w=h, arch=arch)
     rng = np.random.default_rng(seed)
     hist = []
 
@@ -425,13 +426,30 @@ def metrics(fine: RunResult, coarse: RunResult, i: int, nearshore_depth: float =
 
 
 def save(path: str, net: "nn.Net", scale_out, info: dict):
+    info = dict(info, arch=net.arch.as_dict())
     np.savez(path, params=net.params(), scale_out=np.asarray(scale_out, np.float64),
              scale_in=np.asarray(SCALE_IN, np.float64), info=json.dumps(info))
 
 
 def load(path: str = WEIGHTS):
+    """(params, scale_out, info) of a saved network; info["arch"] holds its
+    width and number of middle layers (arch_of)."""
     with np.load(path, allow_pickle=False) as f:
         return f["params"].astype(np.float32), [float(v) for v in f["scale_out"]], json.loads(str(f["info"]))
+
+
+def arch_of(info: dict) -> "nn.Arch":
+    a = info.get("arch", {"width": 32, "mids": 2})
+    return nn.Arch(int(a["width"]), int(a["mids"]))
+
+
+def load_net(path: str = WEIGHTS, max_batch: int = 32, h: int = None, w: int = None):
+    """The saved network ready for inference on h x w images (default: the
+    coarse grid); returns (net, scale_out, info)."""
+    params, scale_out, info = load(path)
+    h = h or COARSE_N
+    net = nn.Net(params, max_batch=max_batch, h=h, w=w or h, arch=arch_of(info))
+    return net, scale_out, info
 
 
 def evaluate(net, scale_out, seeds: Sequence[int], fine_steps: int = 3000, resid_tau: float = 0.5, fine_n: int = FINE_N,
@@ -549,7 +567,7 @@ def evaluate(net, scale_out, seeds: Sequence[int], fine_steps: int = 3000, resid
 def time_network(net: "nn.Net", n: int, reps: int = 10) -> dict:
     """hipEvent-timed forward passes of the network over n images (inputs
     zero, so the timing is data-independent); FLOPs counted as the MFMA
-    work of the four convolutions on the real channel counts."""
+    work of the convolutions on the real channel counts."""
     import torch
     h, w = net.h, net.w
     x = torch.zeros((n, h, w, 8), dtype=torch.bfloat16, device="cuda")
@@ -562,7 +580,7 @@ def time_network(net: "nn.Net", n: int, reps: int = 10) -> dict:
     b.record()
     b.synchronize()
     ms = a.elapsed_time(b) / reps
-    flop_px = 2 * 9 * (8 * 32 + 32 * 32 + 32 * 32 + 32 * 4)
+    flop_px = net.arch.flop_per_pixel
     flops = flop_px * n * h * w
     return {"images": n, "ms": ms, "tflops": flops / (ms * 1e-3) / 1e12, "flop_per_pixel": flop_px}
 
@@ -576,6 +594,8 @@ def main(argv=None):
     ap.add_argument("--epochs", type=int, default=60)
     ap.add_argument("--rounds", type=int, default=0, help="on-trajectory data rounds after the first training")
     ap.add_argument("--batch", type=int, default=8)
+    ap.add_argument("--width", type=int, default=32, choices=[32, 64], help="hidden channels")
+    ap.add_argument("--mids", type=int, default=2, help="hidden 3x3 layers between the input and output ones")
     ap.add_argument("--lr", type=float, default=2e-3)
     ap.add_argument("--heldout", type=int, default=200)
     ap.add_argument("--fine-steps", type=int, default=3000)
@@ -594,7 +614,7 @@ def main(argv=None):
         t_data = time.perf_counter() - t0
         log("[train] %d pairs in %.1f s" % (ds.count, t_data))
         t1 = time.perf_counter()
-        net, hist = train(ds, a.epochs, batch=a.batch, lr=a.lr)
+        net, hist = train(ds, a.epochs, batch=a.batch, lr=a.lr, arch=nn.Arch(a.width, a.mids))
         sets, pairs, hists = [ds], ds.count, [hist]
         # on-trajectory rounds: new scenarios run with the current network
         # applied, so training also sees the states the network itself leads to
@@ -615,8 +635,7 @@ def main(argv=None):
         result["train"] = {k: v for k, v in info.items() if k != "train_seeds"}
         del ds, sets
     if a.cmd in ("evaluate", "all"):
-        params, scale_out, _ = load(a.weights)
-        net = nn.Net(params, max_batch=32, h=COARSE_N, w=COARSE_N)
+        net, scale_out, _ = load_net(a.weights)
         seeds = list(range(100001, 100001 + a.heldout))
         result["network_forward"] = time_network(net, 32)
         log("[eval] network forward: %s" % result["network_forward"])

@@ -17,6 +17,7 @@ enum NnKind {
     NN_WGRAD_IN = 5,    // weight gradient of the input layer
     NN_WGRAD_MID = 6,
     NN_WGRAD_OUT = 7,
+    NN_W64 = 16,        // flag: hidden width 64 (8 -> 64 -> ... -> 64 -> 4)
 };
 
 // Waves of the weight-gradient contraction (partial sums per wave).

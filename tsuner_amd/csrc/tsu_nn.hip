@@ -609,6 +609,21 @@ hipError_t nn_launch_conv(int kind, const uint16_t* in, const uint16_t* w, const
     case NN_DGRAD_MID:   // dy 32 -> dx 32, ReLU mask
         conv_kernel<32, 32, 32, -1, 2><<<grid, block, 0, s>>>(in, w, bias, act, out, N, H, W);
         break;
+    case NN_FWD_IN | NN_W64:
+        conv_kernel<8, 64, 64, 1, 0><<<grid, block, 0, s>>>(in, w, bias, act, out, N, H, W);
+        break;
+    case NN_FWD_MID | NN_W64:
+        conv_kernel<64, 64, 64, 1, 0><<<grid, block, 0, s>>>(in, w, bias, act, out, N, H, W);
+        break;
+    case NN_FWD_OUT | NN_W64:
+        conv_kernel<64, 16, 4, 1, 1><<<grid, block, 0, s>>>(in, w, bias, act, out, N, H, W);
+        break;
+    case NN_DGRAD_OUT | NN_W64:
+        conv_kernel<8, 64, 64, -1, 2><<<grid, block, 0, s>>>(in, w, bias, act, out, N, H, W);
+        break;
+    case NN_DGRAD_MID | NN_W64:
+        conv_kernel<64, 64, 64, -1, 2><<<grid, block, 0, s>>>(in, w, bias, act, out, N, H, W);
+        break;
     default:
         return hipErrorInvalidValue;
     }
@@ -636,6 +651,21 @@ hipError_t nn_launch_wgrad(int kind, const uint16_t* x, const uint16_t* dy, floa
         wgrad_kernel<32, 8><<<grid, block, 0, s>>>(x, dy, part, bpart, N, H, W, waves);
         my = 8;
         kx = 32;
+        break;
+    case NN_WGRAD_IN | NN_W64:
+        wgrad_kernel<8, 64><<<grid, block, 0, s>>>(x, dy, part, bpart, N, H, W, waves);
+        my = 64;
+        kx = 8;
+        break;
+    case NN_WGRAD_MID | NN_W64:
+        wgrad_kernel<64, 64><<<grid, block, 0, s>>>(x, dy, part, bpart, N, H, W, waves);
+        my = 64;
+        kx = 64;
+        break;
+    case NN_WGRAD_OUT | NN_W64:
+        wgrad_kernel<64, 8><<<grid, block, 0, s>>>(x, dy, part, bpart, N, H, W, waves);
+        my = 8;
+        kx = 64;
         break;
     default:
         return hipErrorInvalidValue;

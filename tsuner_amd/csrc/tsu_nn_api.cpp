@@ -22,10 +22,12 @@ bool aligned16(const void* p) { return ((uintptr_t)p & 15) == 0; }
 // rows (MY) and columns (KX) of the weight gradient per kind
 bool wgrad_shape(int kind, int* my, int* kx)
 {
-    switch (kind) {
-    case TSU_NN_WGRAD_IN: *my = 32; *kx = 8; return true;
-    case TSU_NN_WGRAD_MID: *my = 32; *kx = 32; return true;
-    case TSU_NN_WGRAD_OUT: *my = 8; *kx = 32; return true;
+    if (kind & ~(TSU_NN_W64 | 7)) return false;
+    const int width = (kind & TSU_NN_W64) ? 64 : 32;
+    switch (kind & 7) {
+    case TSU_NN_WGRAD_IN: *my = width; *kx = 8; return true;
+    case TSU_NN_WGRAD_MID: *my = width; *kx = width; return true;
+    case TSU_NN_WGRAD_OUT: *my = 8; *kx = width; return true;
     default: return false;
     }
 }
@@ -37,9 +39,10 @@ extern "C" {
 TSU_API int tsu_nn_conv(int kind, const void* in, const void* w, const float* bias, const void* act, void* out, int n,
                         int h, int w_, void* stream)
 {
-    if (kind < TSU_NN_FWD_IN || kind > TSU_NN_DGRAD_MID) return tsu::set_error(TSU_ERR_ARG, "tsu_nn_conv: bad kind");
+    const int base = kind & ~TSU_NN_W64;
+    if (base < TSU_NN_FWD_IN || base > TSU_NN_DGRAD_MID) return tsu::set_error(TSU_ERR_ARG, "tsu_nn_conv: bad kind");
     if (!in || !w || !out || n < 1 || h < 1 || w_ < 1) return tsu::set_error(TSU_ERR_ARG, "tsu_nn_conv: bad argument");
-    const bool needs_bias = kind <= TSU_NN_FWD_OUT, needs_act = kind >= TSU_NN_DGRAD_OUT;
+    const bool needs_bias = base <= TSU_NN_FWD_OUT, needs_act = base >= TSU_NN_DGRAD_OUT;
     if ((needs_bias && !bias) || (needs_act && !act)) return tsu::set_error(TSU_ERR_ARG, "tsu_nn_conv: missing operand");
     if (!aligned16(in) || !aligned16(w) || !aligned16(out) || (act && !aligned16(act)))
         return tsu::set_error(TSU_ERR_ARG, "tsu_nn_conv: operands must be 16-byte aligned");

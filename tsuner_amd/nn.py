@@ -16,25 +16,56 @@ import numpy as np
 from . import _lib
 from ._lib import check, lib
 
-CIN, WIDTH, COUT = 8, 32, 4
-# (rows held, input channels, rows with gradients) of the four weight tensors
-LAYERS = [(WIDTH, CIN), (WIDTH, WIDTH), (WIDTH, WIDTH), (16, WIDTH)]
-WGRAD_KIND = [5, 6, 6, 7]      # TSU_NN_WGRAD_IN, _MID, _MID, _OUT
+CIN, COUT = 8, 4
+W64 = 16                       # TSU_NN_W64: the 64-channel kernel variants
+# kernel kinds (TSU_NN_* in include/tsuner.h)
+FWD_IN, FWD_MID, FWD_OUT, DGRAD_OUT, DGRAD_MID, WGRAD_IN, WGRAD_MID, WGRAD_OUT = range(8)
 
 
-def layout():
-    """Offsets (w, b) of each layer in the flat parameter vector and its size."""
-    offs, o = [], 0
-    for m, k in LAYERS:
-        w = o
-        o += 9 * m * k
-        b = o
-        o += m
-        offs.append((w, b))
-    return offs, o
+class Arch:
+    """Layer shapes of a network 8 -> width (x mids + 1) -> 4: an input
+    convolution, `mids` width -> width convolutions, an output convolution
+    (weights held as 16 rows, 4 used), ReLU after all but the last."""
+
+    def __init__(self, width: int = 32, mids: int = 2):
+        if width not in (32, 64) or mids < 0:
+            raise ValueError("width must be 32 or 64 and mids >= 0")
+        self.width, self.mids = width, mids
+        self.flag = W64 if width == 64 else 0
+        # (rows held, input channels) of each weight tensor
+        self.layers = [(width, CIN)] + [(width, width)] * mids + [(16, width)]
+        self.offs, o = [], 0
+        for m, k in self.layers:
+            w = o
+            o += 9 * m * k
+            b = o
+            o += m
+            self.offs.append((w, b))
+        self.nparam = o
+        self.flop_per_pixel = 2 * 9 * (CIN * width + mids * width * width + width * COUT)
+
+    @property
+    def nlayers(self) -> int:
+        return len(self.layers)
+
+    def wgrad_kind(self, li: int) -> int:
+        k = WGRAD_IN if li == 0 else (WGRAD_OUT if li == self.nlayers - 1 else WGRAD_MID)
+        return k | self.flag
+
+    def as_dict(self) -> dict:
+        return {"width": self.width, "mids": self.mids}
+
+    def __eq__(self, other):
+        return isinstance(other, Arch) and self.as_dict() == other.as_dict()
+
+    def __repr__(self):
+        return "Arch(width=%d, mids=%d)" % (self.width, self.mids)
 
 
-OFFS, NPARAM = layout()
+DEFAULT = Arch(32, 2)
+# the default network's layer table (kept for callers of the fixed shape)
+LAYERS, OFFS, NPARAM = DEFAULT.layers, DEFAULT.offs, DEFAULT.nparam
+WIDTH = DEFAULT.width
 
 
 def _ptr(t):
@@ -46,14 +77,15 @@ def _stream():
     return C.c_void_p(torch.cuda.current_stream().cuda_stream)
 
 
-def init_params(seed: int = 0) -> np.ndarray:
+def init_params(seed: int = 0, arch: Arch = DEFAULT) -> np.ndarray:
     """He-normal weights over the real fan-in, zero biases; the output layer's
     unused rows 4..15 are zero."""
     rng = np.random.default_rng(seed)
-    p = np.zeros(NPARAM, np.float32)
-    for li, ((m, k), (wo, _)) in enumerate(zip(LAYERS, OFFS)):
+    p = np.zeros(arch.nparam, np.float32)
+    last = arch.nlayers - 1
+    for li, ((m, k), (wo, _)) in enumerate(zip(arch.layers, arch.offs)):
         w = rng.normal(0.0, math.sqrt(2.0 / (9 * k)), size=(9, m, k)).astype(np.float32)
-        if li == 3:
+        if li == last:
             w[:, COUT:, :] = 0.0
             w *= 0.1           # start near the identity correction (zero output)
         p[wo:wo + w.size] = w.ravel()
@@ -65,30 +97,31 @@ class Net:
     `max_batch` images of h x w pixels."""
 
     def __init__(self, params: np.ndarray = None, max_batch: int = 8, h: int = 256, w: int = 256,
-                 device: int = 0):
+                 device: int = 0, arch: Arch = DEFAULT):
         import torch
         self.torch = torch
+        self.arch = arch
         self.dev = torch.device("cuda", device)
         self.h, self.w, self.max_batch = h, w, max_batch
-        p = init_params() if params is None else np.asarray(params, np.float32)
-        assert p.size == NPARAM
+        p = init_params(arch=arch) if params is None else np.asarray(params, np.float32)
+        assert p.size == arch.nparam, "parameter vector does not match %r" % (arch,)
         f32 = dict(dtype=torch.float32, device=self.dev)
         self.p = torch.from_numpy(p.copy()).to(self.dev)
-        self.g = torch.zeros(NPARAM, **f32)
-        self.m = torch.zeros(NPARAM, **f32)
-        self.v = torch.zeros(NPARAM, **f32)
+        self.g = torch.zeros(arch.nparam, **f32)
+        self.m = torch.zeros(arch.nparam, **f32)
+        self.v = torch.zeros(arch.nparam, **f32)
         self.t = 0
         bf = dict(dtype=torch.bfloat16, device=self.dev)
-        self.wb = [torch.zeros(9 * m * k, **bf) for m, k in LAYERS]
-        # per-tap transposes used by the data gradients of layers 2, 3, 4
-        self.wt = [None, torch.zeros(9 * WIDTH * WIDTH, **bf), torch.zeros(9 * WIDTH * WIDTH, **bf),
-                   torch.zeros(9 * WIDTH * 8, **bf)]
+        W, L = arch.width, arch.nlayers
+        self.wb = [torch.zeros(9 * m * k, **bf) for m, k in arch.layers]
+        # per-tap transposes used by the data gradients of layers 1 .. L-1
+        self.wt = [None] + [torch.zeros(9 * W * W, **bf) for _ in range(arch.mids)] + [torch.zeros(9 * W * 8, **bf)]
         npix = max_batch * h * w
-        self.acts = [torch.zeros(npix * WIDTH, **bf) for _ in range(3)]
+        self.acts = [torch.zeros(npix * W, **bf) for _ in range(L - 1)]
         self.out = torch.zeros(npix * COUT, **f32)
         self.dy4 = torch.zeros(npix * 8, **bf)
-        self.dbuf = [torch.zeros(npix * WIDTH, **bf) for _ in range(2)]
-        self.ws = torch.zeros(max(lib().tsu_nn_wgrad_workspace(k) for k in (5, 6, 7)), **f32)
+        self.dbuf = [torch.zeros(npix * W, **bf) for _ in range(2)]
+        self.ws = torch.zeros(max(lib().tsu_nn_wgrad_workspace(arch.wgrad_kind(li)) for li in range(L)), **f32)
         self.nblocks = 1024
         self.loss_part = torch.zeros(self.nblocks, dtype=torch.float64, device=self.dev)
         self.pack()
@@ -96,18 +129,19 @@ class Net:
     # ---- parameters ------------------------------------------------------
     def pack(self):
         L, s = lib(), _stream()
-        for li, (m, k) in enumerate(LAYERS):
-            wo, _ = OFFS[li]
+        last = self.arch.nlayers - 1
+        for li, (m, k) in enumerate(self.arch.layers):
+            wo, _ = self.arch.offs[li]
             wp = C.c_void_p(self.p.data_ptr() + 4 * wo)
             wt = _ptr(self.wt[li]) if self.wt[li] is not None else None
-            mt = 8 if li == 3 else m
+            mt = 8 if li == last else m
             check(L.tsu_nn_pack(wp, _ptr(self.wb[li]), wt, m, k, mt, s), "tsu_nn_pack")
 
     def params(self) -> np.ndarray:
         return self.p.detach().cpu().numpy().copy()
 
     def _bias(self, li):
-        return C.c_void_p(self.p.data_ptr() + 4 * OFFS[li][1])
+        return C.c_void_p(self.p.data_ptr() + 4 * self.arch.offs[li][1])
 
     # ---- forward / backward ----------------------------------------------
     def forward(self, x, n: int, out=None):
@@ -116,23 +150,24 @@ class Net:
         the network's own buffer, which is returned."""
         assert 1 <= n <= self.max_batch
         L, s = lib(), _stream()
+        f, last = self.arch.flag, self.arch.nlayers - 1
         xin = C.c_void_p(x if isinstance(x, int) else x.data_ptr())
         dst = self.out if out is None else out
         dst_addr = dst if isinstance(dst, int) else dst.data_ptr()
         h, w = self.h, self.w
-        check(L.tsu_nn_conv(0, xin, _ptr(self.wb[0]), self._bias(0), None, _ptr(self.acts[0]), n, h, w, s), "conv1")
-        check(L.tsu_nn_conv(1, _ptr(self.acts[0]), _ptr(self.wb[1]), self._bias(1), None, _ptr(self.acts[1]),
-                            n, h, w, s), "conv2")
-        check(L.tsu_nn_conv(1, _ptr(self.acts[1]), _ptr(self.wb[2]), self._bias(2), None, _ptr(self.acts[2]),
-                            n, h, w, s), "conv3")
-        check(L.tsu_nn_conv(2, _ptr(self.acts[2]), _ptr(self.wb[3]), self._bias(3), None, C.c_void_p(dst_addr),
-                            n, h, w, s), "conv4")
+        src = xin
+        for li in range(last + 1):
+            kind = (FWD_IN if li == 0 else FWD_OUT if li == last else FWD_MID) | f
+            o = C.c_void_p(dst_addr) if li == last else _ptr(self.acts[li])
+            check(L.tsu_nn_conv(kind, src, _ptr(self.wb[li]), self._bias(li), None, o, n, h, w, s), "conv%d" % li)
+            src = o
         return dst
 
     def backward(self, x, target, n: int):
         """Gradients of the mean-squared error into self.g; returns the loss
         partial-sum buffer (sum of squared errors, not yet divided)."""
         L, s = lib(), _stream()
+        A = self.arch
         h, w = self.h, self.w
         npix = n * h * w
         xin = C.c_void_p(x if isinstance(x, int) else x.data_ptr())
@@ -140,16 +175,17 @@ class Net:
         check(L.tsu_nn_mse(_ptr(self.out), tin, _ptr(self.dy4), _ptr(self.loss_part), self.nblocks, npix,
                            C.c_float(2.0 / (4.0 * npix)), s), "mse")
         ws, wsn = _ptr(self.ws), self.ws.numel()
-        inputs = [xin, _ptr(self.acts[0]), _ptr(self.acts[1]), _ptr(self.acts[2])]
+        inputs = [xin] + [_ptr(a) for a in self.acts]
         dy = _ptr(self.dy4)
-        for li in (3, 2, 1, 0):
-            wo, bo = OFFS[li]
-            m = LAYERS[li][0]
-            check(L.tsu_nn_wgrad(WGRAD_KIND[li], inputs[li], dy, C.c_void_p(self.g.data_ptr() + 4 * wo),
+        last = A.nlayers - 1
+        for li in range(last, -1, -1):
+            wo, bo = A.offs[li]
+            m = A.layers[li][0]
+            check(L.tsu_nn_wgrad(A.wgrad_kind(li), inputs[li], dy, C.c_void_p(self.g.data_ptr() + 4 * wo),
                                  C.c_void_p(self.g.data_ptr() + 4 * bo), m, n, h, w, ws, wsn, s), "wgrad%d" % li)
             if li > 0:
                 dst = self.dbuf[li % 2]
-                kind = 3 if li == 3 else 4
+                kind = (DGRAD_OUT if li == last else DGRAD_MID) | A.flag
                 check(L.tsu_nn_conv(kind, dy, _ptr(self.wt[li]), None, inputs[li], _ptr(dst), n, h, w, s),
                       "dgrad%d" % li)
                 dy = _ptr(dst)
@@ -158,7 +194,7 @@ class Net:
     def adam(self, lr: float, b1: float = 0.9, b2: float = 0.999, eps: float = 1e-8):
         self.t += 1
         lr_t = lr * math.sqrt(1.0 - b2 ** self.t) / (1.0 - b1 ** self.t)
-        check(lib().tsu_nn_adam(_ptr(self.p), _ptr(self.g), _ptr(self.m), _ptr(self.v), NPARAM, C.c_float(lr_t),
+        check(lib().tsu_nn_adam(_ptr(self.p), _ptr(self.g), _ptr(self.m), _ptr(self.v), self.arch.nparam, C.c_float(lr_t),
                                 C.c_float(b1), C.c_float(b2), C.c_float(eps), _stream()), "adam")
         self.pack()
 
