@@ -188,7 +188,8 @@ def run_correction(heldout, fine_steps):
     tn = K.time_network(net, 32)
     log("[bench] correction network: %.3f ms per 32 images, %.1f TFLOP/s" % (tn["ms"], tn["tflops"]))
     ev = K.evaluate(net, scale_out, list(range(300001, 300001 + heldout)), fine_steps=fine_steps)
-    keys = ("peak_eta_err_mean_m", "frac_peak_eta_within_5cm", "nearshore_mean_abs_eta_err_m", "runup_rel_err_mean",
+    keys = ("peak_eta_err_mean_m", "frac_peak_eta_within_5cm", "peak_eta_sea_err_mean_m",
+            "frac_peak_eta_sea_within_5cm", "nearshore_mean_abs_eta_err_m", "runup_rel_err_mean",
             "frac_runup_within_10pct", "sims_per_hr", "speedup_vs_fine", "fallback_frac")
 
     def pick(d):

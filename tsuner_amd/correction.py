@@ -407,8 +407,16 @@ def metrics(fine: RunResult, coarse: RunResult, i: int, nearshore_depth: float =
     ec = coarse.eta_max[i]
     zone = coarse.H0[i] < nearshore_depth
     both = zone & np.isfinite(ef) & np.isfinite(ec)
-    peak_f = float(np.nanmax(np.where(zone, ef, np.nan))) if np.isfinite(ef[zone]).any() else float("nan")
-    peak_c = float(np.nanmax(np.where(zone, ec, np.nan))) if np.isfinite(ec[zone]).any() else float("nan")
+
+    def peak(a, z):
+        return float(np.nanmax(np.where(z, a, np.nan))) if np.isfinite(a[z]).any() else float("nan")
+
+    peak_f, peak_c = peak(ef, zone), peak(ec, zone)
+    # the same over the sea part of the zone only (initially wet, depth
+    # below nearshore_depth): the coastal wave height without the
+    # inundation front, whose coarse surface sits on block-mean ground
+    sea = zone & (coarse.H0[i] > 0)
+    peak_fs, peak_cs = peak(ef, sea), peak(ec, sea)
     # runup: the fine run's highest wetted fine ground against the coarse
     # run's sub-grid reading of the same fine ground; the coarse run's own
     # highest wetted (block-mean) ground is reported beside it
@@ -420,6 +428,7 @@ def metrics(fine: RunResult, coarse: RunResult, i: int, nearshore_depth: float =
         return abs(a - b) / b if b > 0 else (0.0 if a == 0 else float("inf"))
 
     return {"peak_eta_fine": peak_f, "peak_eta_coarse": peak_c, "peak_eta_err": abs(peak_c - peak_f),
+            "peak_eta_sea_fine": peak_fs, "peak_eta_sea_coarse": peak_cs, "peak_eta_sea_err": abs(peak_cs - peak_fs),
             "mean_abs_eta_err": float(np.mean(np.abs(ec[both] - ef[both]))) if both.any() else float("nan"),
             "runup_fine": rf, "runup_coarse": rc, "runup_rel_err": rel(rc, rf),
             "runup_coarse_ground": rcg, "runup_rel_err_coarse_ground": rel(rcg, rf)}
@@ -479,6 +488,9 @@ def evaluate(net, scale_out, seeds: Sequence[int], fine_steps: int = 3000, resid
 
     def summary(ms, use_fine=None):
         pe = np.array([0.0 if (use_fine is not None and use_fine[i]) else m["peak_eta_err"] for i, m in enumerate(ms)])
+        ps = np.array([0.0 if (use_fine is not None and use_fine[i]) else m["peak_eta_sea_err"]
+                       for i, m in enumerate(ms)])
+        oks = np.isfinite(ps)
         me = np.array([0.0 if (use_fine is not None and use_fine[i]) else m["mean_abs_eta_err"] for i, m in enumerate(ms)])
         ru = np.array([0.0 if (use_fine is not None and use_fine[i]) else m["runup_rel_err"] for i, m in enumerate(ms)])
         rf = np.array([0.0 if (use_fine is not None and use_fine[i]) else m["runup_rel_err_coarse_ground"]
@@ -497,6 +509,8 @@ def evaluate(net, scale_out, seeds: Sequence[int], fine_steps: int = 3000, resid
                 "runup_rel_err_p95": stat(lambda a: np.percentile(a, 95), ru[okr]),
                 "runup_rel_err_coarse_ground_mean": stat(np.mean, rf[np.isfinite(rf)]),
                 "frac_peak_eta_within_5cm": stat(np.mean, (pe[ok] <= 0.05).astype(float)),
+                "peak_eta_sea_err_mean_m": stat(np.mean, ps[oks]),
+                "frac_peak_eta_sea_within_5cm": stat(np.mean, (ps[oks] <= 0.05).astype(float)),
                 "frac_runup_within_10pct": stat(np.mean, (ru[okr] <= 0.10).astype(float)),
                 "scenarios_with_runup": int(sum(m["runup_fine"] > 0 for m in ms)),
                 "scenarios_with_peak": int(ok.sum())}
@@ -556,6 +570,10 @@ def evaluate(net, scale_out, seeds: Sequence[int], fine_steps: int = 3000, resid
                            "peak_eta_fine": [m["peak_eta_fine"] for m in mc],
                            "peak_eta_err_coarse": [m["peak_eta_err"] for m in mb],
                            "peak_eta_err_nn": [m["peak_eta_err"] for m in mc],
+                           "peak_eta_coarse": [m["peak_eta_coarse"] for m in mb],
+                           "peak_eta_nn": [m["peak_eta_coarse"] for m in mc],
+                           "peak_eta_sea_fine": [m["peak_eta_sea_fine"] for m in mc],
+                           "peak_eta_sea_nn": [m["peak_eta_sea_coarse"] for m in mc],
                            "runup_fine": [m["runup_fine"] for m in mc],
                            "runup_rel_err_coarse": [m["runup_rel_err"] for m in mb],
                            "runup_rel_err_nn": [m["runup_rel_err"] for m in mc],
