@@ -606,7 +606,8 @@ def time_network(net: "nn.Net", n: int, reps: int = 10) -> dict:
 def main(argv=None):
     import argparse
     ap = argparse.ArgumentParser(description="learned coarse-grid correction: train / evaluate")
-    ap.add_argument("cmd", choices=["train", "evaluate", "all"])
+    ap.add_argument("cmd", choices=["train", "evaluate", "all", "network"],
+                    help="network: time the saved network's forward pass on 32 coarse images")
     ap.add_argument("--seeds", type=int, default=48, help="training scenarios")
     ap.add_argument("--windows", type=int, default=75, help="windows per training scenario")
     ap.add_argument("--epochs", type=int, default=60)
@@ -652,6 +653,9 @@ def main(argv=None):
         save(a.weights, net, scale_out, info)
         result["train"] = {k: v for k, v in info.items() if k != "train_seeds"}
         del ds, sets
+    if a.cmd == "network":
+        net, _, _ = load_net(a.weights)
+        result["network_forward"] = time_network(net, 32, reps=20)
     if a.cmd in ("evaluate", "all"):
         net, scale_out, _ = load_net(a.weights)
         seeds = list(range(100001, 100001 + a.heldout))

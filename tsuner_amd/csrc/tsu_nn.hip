@@ -54,15 +54,28 @@ __device__ __forceinline__ int tap_dx(int t) { return t % 3 - 1; }
 constexpr int kConvRows = 4;
 constexpr int kTileRows = kConvRows + 2, kTileCols = 64 + 2;
 
+// LDS slot of octet q of tile pixel p: the 16 lanes of a ds_read_b128 read
+// the same octet of 16 consecutive pixels, which with KO octets per pixel
+// are KO*16 bytes apart and would share banks; XOR-ing q with the pixel's
+// bits above log2(16/KO) spreads them over all 64 banks.
+template <int KO>
+__device__ __forceinline__ int swz(int p, int q)
+{
+    return q ^ ((p / (16 / KO)) & (KO - 1));
+}
+
 template <int K, int M, int MOUT, int SIGN, int POST>
-__global__ __launch_bounds__(256) void conv_kernel(const uint16_t* __restrict__ in, const uint16_t* __restrict__ Wt,
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) void conv_kernel(const uint16_t* __restrict__ in, const uint16_t* __restrict__ Wt,
                                                    const float* __restrict__ bias, const uint16_t* __restrict__ act,
                                                    void* __restrict__ out, int N, int H, int W)
 {
     constexpr int KO = K / 8;                  // octets per tap
     constexpr int STEPS = (9 * KO + 3) / 4;    // MFMAs per (m-tile, n-tile)
     constexpr int MT = M / 16;
-    __shared__ uint4 tile[kTileRows * kTileCols * KO];
+    constexpr int TILE = kTileRows * kTileCols * KO;
+    constexpr int PER_THREAD = (TILE + 255) / 256;
+    constexpr int AHEAD = 2;                   // weight operands prefetched this many steps ahead
+    __shared__ uint4 tile[TILE];
     const int segs = (W + 63) / 64, rblocks = (H + kConvRows - 1) / kConvRows;
     long long blk = blockIdx.x;
     const int seg = (int)(blk % segs);
@@ -71,19 +84,52 @@ __global__ __launch_bounds__(256) void conv_kernel(const uint16_t* __restrict__ 
     const long long n = blk / rblocks;
     if (n >= N) return;
     const int c0 = seg * 64;
-    for (int idx = threadIdx.x; idx < kTileRows * kTileCols * KO; idx += 256) {
-        const int q = idx % KO, rest = idx / KO;
-        const int gc = c0 - 1 + rest % kTileCols, gr = r0 - 1 + rest / kTileCols;
-        uint4 v = make_uint4(0, 0, 0, 0);
-        if (gr >= 0 && gr < H && gc >= 0 && gc < W) v = *(const uint4*)(in + (((size_t)n * H + gr) * W + gc) * K + 8 * q);
-        tile[idx] = v;
+    const int wv = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    const int g = lane >> 4, li = lane & 15;
+
+    // weight operands of step s: A[a] = Wt[t][a*16 + li][8q .. 8q+7] for this
+    // lane's octet o = 4s + g (t = o / KO, q = o % KO; zero past the 9 taps)
+    auto load_a = [&](int s, bf16x8* A) {
+        const int o = 4 * s + g;
+        const int t = o / KO, q = o % KO;
+#pragma unroll
+        for (int a = 0; a < MT; a++) {
+            uint4 v = make_uint4(0, 0, 0, 0);
+            if (t < 9) v = *(const uint4*)(Wt + ((size_t)(t * M + a * 16 + li) * K + 8 * q));
+            A[a] = as_bf16x8(v);
+        }
+    };
+    // the first steps' weights are in flight while the input tile is staged
+    bf16x8 Abuf[AHEAD + 1][MT];
+#pragma unroll
+    for (int s = 0; s < AHEAD && s < STEPS; s++) load_a(s, Abuf[s]);
+
+    // input tile, in batches of up to 7 loads per thread, each batch's loads
+    // issued before its LDS stores
+    constexpr int BATCH = PER_THREAD < 7 ? PER_THREAD : 7;
+#pragma unroll
+    for (int j0 = 0; j0 < PER_THREAD; j0 += BATCH) {
+        uint4 stage[BATCH];
+#pragma unroll
+        for (int j = 0; j < BATCH; j++) {
+            const int idx = threadIdx.x + 256 * (j0 + j);
+            const int q = idx % KO, rest = idx / KO;
+            const int gc = c0 - 1 + rest % kTileCols, gr = r0 - 1 + rest / kTileCols;
+            stage[j] = make_uint4(0, 0, 0, 0);
+            if (j0 + j < PER_THREAD && idx < TILE && gr >= 0 && gr < H && gc >= 0 && gc < W)
+                stage[j] = *(const uint4*)(in + (((size_t)n * H + gr) * W + gc) * K + 8 * q);
+        }
+#pragma unroll
+        for (int j = 0; j < BATCH; j++) {
+            const int idx = threadIdx.x + 256 * (j0 + j);
+            const int q = idx % KO, rest = idx / KO;
+            if (j0 + j < PER_THREAD && idx < TILE) tile[rest * KO + swz<KO>(rest, q)] = stage[j];
+        }
     }
     __syncthreads();
-    const int wv = threadIdx.x >> 6, lane = threadIdx.x & 63;
     const int r = r0 + wv;
     if (r >= H) return;                        // no barrier follows
     const long long row = n * H + r;
-    const int g = lane >> 4, li = lane & 15;
 
     f32x4 acc[MT][4];
 #pragma unroll
@@ -93,22 +139,17 @@ __global__ __launch_bounds__(256) void conv_kernel(const uint16_t* __restrict__ 
 
 #pragma unroll
     for (int s = 0; s < STEPS; s++) {
+        if (s + AHEAD < STEPS) load_a(s + AHEAD, Abuf[(s + AHEAD) % (AHEAD + 1)]);
+        const bf16x8* A = Abuf[s % (AHEAD + 1)];
         const int o = 4 * s + g;               // this lane's octet
         const int t = o / KO, q = o % KO;
         const bool tv = t < 9;
         const int dy = SIGN * tap_dy(t), dx = SIGN * tap_dx(t);
-        bf16x8 A[MT];
-#pragma unroll
-        for (int a = 0; a < MT; a++) {
-            uint4 v = make_uint4(0, 0, 0, 0);
-            if (tv) v = *(const uint4*)(Wt + ((size_t)(t * M + a * 16 + li) * K + 8 * q));
-            A[a] = as_bf16x8(v);
-        }
         const int lrow = (wv + 1 + dy) * kTileCols + 1 + dx + li;
 #pragma unroll
         for (int b = 0; b < 4; b++) {
             uint4 v = make_uint4(0, 0, 0, 0);
-            if (tv) v = tile[(lrow + b * 16) * KO + q];
+            if (tv) v = tile[(lrow + b * 16) * KO + swz<KO>(lrow + b * 16, q)];
             const bf16x8 B = as_bf16x8(v);
 #pragma unroll
             for (int a = 0; a < MT; a++)
