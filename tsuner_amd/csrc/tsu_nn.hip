@@ -197,6 +197,174 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) void c
     }
 }
 
+// Convolutions of the wide (64-channel) network: 64 -> 64 (middle layers
+// and their data gradients), 8 -> 64 (input layer) and 64 -> 4 (output
+// layer).  Same arithmetic and operand layout as conv_kernel, arranged so
+// that memory latency stops bounding it:
+//   * the layer's weights are staged in LDS once per workgroup, as
+//     [tap][octet][output channel] 16-byte rows, so a step's A operands are
+//     LDS reads (16 lanes read 16 consecutive rows);
+//   * a workgroup of 8 waves covers 8 rows x 64 columns, one row per wave
+//     (two waves per SIMD hide each other's LDS latency);
+//   * workgroups are persistent (one per CU: for 64 -> 64 the weights and
+//     the 10 x 66 pixel input tile fill 155 KB of the 160 KB LDS) and load
+//     the next tile's input into registers while computing the current one.
+constexpr int kWideRows = 8;
+constexpr int kWideTileRows = kWideRows + 2;
+
+template <int K, int M, int MOUT, int SIGN, int POST>
+__global__ __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(2, 2))) void conv_wide_kernel(
+    const uint16_t* __restrict__ in, const uint16_t* __restrict__ Wt, const float* __restrict__ bias,
+    const uint16_t* __restrict__ act, void* __restrict__ out, int N, int H, int W)
+{
+    constexpr int KO = K / 8, MT = M / 16;
+    constexpr int STEPS = (9 * KO + 3) / 4;
+    constexpr bool TAP_UNIFORM = KO >= 4;                    // a step's 4 octets lie in one tap
+    constexpr int TILE = kWideTileRows * kTileCols * KO;
+    constexpr int WSLOTS = 9 * KO * M;
+    constexpr int PER_THREAD = (TILE + 511) / 512;
+    constexpr int WPER = (WSLOTS + 511) / 512;
+    constexpr int SLOTS = PER_THREAD > WPER ? PER_THREAD : WPER;
+    __shared__ uint4 wl[WSLOTS];
+    __shared__ uint4 tile[TILE];
+    const int segs = (W + 63) / 64, rblocks = (H + kWideRows - 1) / kWideRows;
+    const long long nblk = (long long)N * rblocks * segs;
+    const int wv = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    const int g = lane >> 4, li = lane & 15;
+    if ((long long)blockIdx.x >= nblk) return;              // whole workgroup, before any barrier
+
+    uint4 stage[SLOTS];
+    // weights: global [tap][m][k] -> LDS [tap][octet][m]
+#pragma unroll
+    for (int j = 0; j < WPER; j++) {
+        const int idx = threadIdx.x + 512 * j;              // over (tap, m, octet), octet fastest
+        if (idx < WSLOTS) stage[j] = *(const uint4*)(Wt + (size_t)idx * 8);
+    }
+#pragma unroll
+    for (int j = 0; j < WPER; j++) {
+        const int idx = threadIdx.x + 512 * j;
+        const int q = idx % KO, m = (idx / KO) % M, t = idx / (KO * M);
+        if (idx < WSLOTS) wl[(t * KO + q) * M + m] = stage[j];
+    }
+
+    auto origin = [&](long long b, long long& n, int& r0, int& c0) {
+        c0 = (int)(b % segs) * 64;
+        b /= segs;
+        r0 = (int)(b % rblocks) * kWideRows;
+        n = b / rblocks;
+    };
+    auto fetch = [&](long long b) {
+        long long n;
+        int r0, c0;
+        origin(b, n, r0, c0);
+#pragma unroll
+        for (int j = 0; j < PER_THREAD; j++) {
+            const int idx = threadIdx.x + 512 * j;
+            const int q = idx % KO, rest = idx / KO;
+            const int gc = c0 - 1 + rest % kTileCols, gr = r0 - 1 + rest / kTileCols;
+            stage[j] = make_uint4(0, 0, 0, 0);
+            if (idx < TILE && gr >= 0 && gr < H && gc >= 0 && gc < W)
+                stage[j] = *(const uint4*)(in + (((size_t)n * H + gr) * W + gc) * K + 8 * q);
+        }
+    };
+    auto put = [&]() {
+#pragma unroll
+        for (int j = 0; j < PER_THREAD; j++) {
+            const int idx = threadIdx.x + 512 * j;
+            const int q = idx % KO, rest = idx / KO;
+            if (idx < TILE) tile[rest * KO + swz<KO>(rest, q)] = stage[j];
+        }
+    };
+
+    long long blk = blockIdx.x;
+    fetch(blk);
+    put();
+    __syncthreads();
+    for (;;) {
+        const long long next = blk + gridDim.x;
+        const bool more = next < nblk;                       // uniform over the workgroup
+        if (more) fetch(next);                               // in flight during the MFMAs
+        long long n;
+        int r0, c0;
+        origin(blk, n, r0, c0);
+        f32x4 acc[MT][4];
+#pragma unroll
+        for (int a = 0; a < MT; a++)
+#pragma unroll
+            for (int b = 0; b < 4; b++) acc[a][b] = f32x4{0.f, 0.f, 0.f, 0.f};
+        // unrolled by 2 only: hoisting every step's LDS reads would exhaust
+        // registers
+#pragma unroll 2
+        for (int s = 0; s < STEPS; s++) {
+            int t, q;
+            if (TAP_UNIFORM) {
+                t = (4 * s) / KO;
+                q = (4 * s) % KO + g;
+            } else {
+                t = (4 * s + g) / KO;
+                q = (4 * s + g) % KO;
+            }
+            const bool tv = t < 9;
+            const int dy = SIGN * tap_dy(t), dx = SIGN * tap_dx(t);
+            bf16x8 A[MT];
+#pragma unroll
+            for (int a = 0; a < MT; a++)
+                A[a] = as_bf16x8(tv ? wl[(t * KO + q) * M + a * 16 + li] : make_uint4(0, 0, 0, 0));
+#pragma unroll
+            for (int b = 0; b < 4; b++) {
+                const int p = (wv + 1 + dy) * kTileCols + 1 + dx + b * 16 + li;
+                const bf16x8 B = as_bf16x8(tv ? tile[p * KO + swz<KO>(p, q)] : make_uint4(0, 0, 0, 0));
+#pragma unroll
+                for (int a = 0; a < MT; a++)
+                    acc[a][b] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(A[a], B, acc[a][b], 0, 0, 0);
+            }
+        }
+#pragma unroll
+        for (int b = 0; b < 4; b++) {
+            const int r = r0 + wv, c = c0 + b * 16 + li;
+            if (r >= H || c >= W) continue;
+            const size_t p = ((size_t)n * H + r) * W + c;
+#pragma unroll
+            for (int a = 0; a < MT; a++) {
+                const int m0 = a * 16 + 4 * g;
+                if (m0 >= MOUT) continue;
+                float v[4];
+#pragma unroll
+                for (int i = 0; i < 4; i++) v[i] = acc[a][b][i];
+                if (POST == 0 || POST == 1) {
+#pragma unroll
+                    for (int i = 0; i < 4; i++) v[i] += bias[m0 + i];
+                }
+                if (POST == 0) {
+#pragma unroll
+                    for (int i = 0; i < 4; i++) v[i] = fmaxf(v[i], 0.f);
+                }
+                if (POST == 1) {
+                    *(float4*)((float*)out + p * MOUT + m0) = make_float4(v[0], v[1], v[2], v[3]);
+                } else {
+                    if (POST == 2) {
+                        const uint2 av = *(const uint2*)(act + p * MOUT + m0);
+                        // ReLU outputs are >= 0: only +0 / -0 bit patterns mean inactive
+                        if ((av.x & 0x7FFFu) == 0) v[0] = 0.f;
+                        if ((av.x & 0x7FFF0000u) == 0) v[1] = 0.f;
+                        if ((av.y & 0x7FFFu) == 0) v[2] = 0.f;
+                        if ((av.y & 0x7FFF0000u) == 0) v[3] = 0.f;
+                    }
+                    uint2 st;
+                    st.x = (uint32_t)to_bf16(v[0]) | ((uint32_t)to_bf16(v[1]) << 16);
+                    st.y = (uint32_t)to_bf16(v[2]) | ((uint32_t)to_bf16(v[3]) << 16);
+                    *(uint2*)((uint16_t*)out + p * MOUT + m0) = st;
+                }
+            }
+        }
+        __syncthreads();                                     // every wave is done reading the tile
+        if (!more) break;
+        put();
+        __syncthreads();
+        blk = next;
+    }
+}
+
 // Weight gradient of out = conv(x, w): dW[t][m][k] = sum_p dy[p][m] x[p + off(t)][k],
 // contracted over pixels on the MFMA K axis (32 pixels per MFMA: four runs of
 // 8 consecutive columns of one row).  Grid: (WAVES/4, 9 taps); wave w of tap t
@@ -629,6 +797,23 @@ hipError_t nn_launch_target(const StageArgs* fine, const StageArgs* coarse, int 
     return hipGetLastError();
 }
 
+namespace {
+// conv_wide_kernel's persistent grid: one workgroup per CU (its LDS holds one),
+// at most one per tile.
+dim3 wide_grid(int N, int H, int W)
+{
+    static int cus = 0;
+    if (cus == 0) {
+        int dev = 0;
+        if (hipGetDevice(&dev) != hipSuccess ||
+            hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess || cus < 1)
+            cus = 256;
+    }
+    const long long tiles = (long long)N * ((H + kWideRows - 1) / kWideRows) * ((W + 63) / 64);
+    return dim3((unsigned)(tiles < cus ? tiles : cus));
+}
+}  // namespace
+
 hipError_t nn_launch_conv(int kind, const uint16_t* in, const uint16_t* w, const float* bias, const uint16_t* act,
                           void* out, int N, int H, int W, hipStream_t s)
 {
@@ -651,19 +836,19 @@ hipError_t nn_launch_conv(int kind, const uint16_t* in, const uint16_t* w, const
         conv_kernel<32, 32, 32, -1, 2><<<grid, block, 0, s>>>(in, w, bias, act, out, N, H, W);
         break;
     case NN_FWD_IN | NN_W64:
-        conv_kernel<8, 64, 64, 1, 0><<<grid, block, 0, s>>>(in, w, bias, act, out, N, H, W);
+        conv_wide_kernel<8, 64, 64, 1, 0><<<wide_grid(N, H, W), 512, 0, s>>>(in, w, bias, act, out, N, H, W);
         break;
     case NN_FWD_MID | NN_W64:
-        conv_kernel<64, 64, 64, 1, 0><<<grid, block, 0, s>>>(in, w, bias, act, out, N, H, W);
+        conv_wide_kernel<64, 64, 64, 1, 0><<<wide_grid(N, H, W), 512, 0, s>>>(in, w, bias, act, out, N, H, W);
         break;
     case NN_FWD_OUT | NN_W64:
-        conv_kernel<64, 16, 4, 1, 1><<<grid, block, 0, s>>>(in, w, bias, act, out, N, H, W);
+        conv_wide_kernel<64, 16, 4, 1, 1><<<wide_grid(N, H, W), 512, 0, s>>>(in, w, bias, act, out, N, H, W);
         break;
     case NN_DGRAD_OUT | NN_W64:
-        conv_kernel<8, 64, 64, -1, 2><<<grid, block, 0, s>>>(in, w, bias, act, out, N, H, W);
+        conv_wide_kernel<8, 64, 64, -1, 2><<<wide_grid(N, H, W), 512, 0, s>>>(in, w, bias, act, out, N, H, W);
         break;
     case NN_DGRAD_MID | NN_W64:
-        conv_kernel<64, 64, 64, -1, 2><<<grid, block, 0, s>>>(in, w, bias, act, out, N, H, W);
+        conv_wide_kernel<64, 64, 64, -1, 2><<<wide_grid(N, H, W), 512, 0, s>>>(in, w, bias, act, out, N, H, W);
         break;
     default:
         return hipErrorInvalidValue;
