@@ -153,6 +153,30 @@ def test_mfma_weight_gradients(width):
         np.testing.assert_allclose(host(db), rdb, rtol=0, atol=1e-5 * np.abs(rdb).max())
 
 
+def test_wide_conv_workgroups_loop_over_tiles():
+    """The 64-channel kernels keep one workgroup per CU and loop over 8 x 64
+    pixel tiles (prefetching the next one): an image set with more than
+    twice as many tiles as CUs, ragged in both directions, against the
+    oracle, forward and data gradient."""
+    rng = np.random.default_rng(9)
+    cus = torch.cuda.get_device_properties(0).multi_processor_count
+    h, w = 45, 150                      # 6 x 3 tiles per image, partial ones at both edges
+    n = (2 * cus) // 18 + 1
+    s = C.c_void_p(0)
+    x = rand_bf16(rng, (n, h, w, 64))
+    wt = rand_bf16(rng, (9, 64, 64), 0.1)
+    b = rng.normal(0, 0.1, size=64).astype(np.float32)
+    out = torch.zeros(n * h * w * 64, dtype=torch.bfloat16, device="cuda")
+    ok(lib().tsu_nn_conv(1 | W64, P(dev_bf16(x)), P(dev_bf16(wt)), P(dev_f32(b)), None, P(out), n, h, w, s))
+    assert_bf16_close(host(out).reshape(n, h, w, 64), O.bf16(np.maximum(O.conv(x, wt, b), 0).astype(np.float32)))
+    act = np.maximum(rand_bf16(rng, (n, h, w, 64)), 0)
+    wtr = torch.zeros(9 * 64 * 64, dtype=torch.bfloat16, device="cuda")
+    ok(lib().tsu_nn_pack(P(dev_f32(wt)), P(torch.zeros_like(wtr)), P(wtr), 64, 64, 64, s))
+    ok(lib().tsu_nn_conv(4 | W64, P(dev_bf16(x)), P(wtr), None, P(dev_bf16(act)), P(out), n, h, w, s))
+    ref = O.conv_input_grad(x, wt) * (act > 0)
+    assert_bf16_close(host(out).reshape(n, h, w, 64), O.bf16(ref.astype(np.float32)))
+
+
 def test_mse_and_adam_kernels():
     rng = np.random.default_rng(4)
     npix = 1000
