@@ -200,10 +200,14 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) void c
 // Convolutions of the wide (64-channel) network: 64 -> 64 (middle layers
 // and their data gradients), 8 -> 64 (input layer) and 64 -> 4 (output
 // layer).  Same arithmetic and operand layout as conv_kernel, arranged so
-// that memory latency stops bounding it:
+// that neither memory latency nor address arithmetic bounds it:
 //   * the layer's weights are staged in LDS once per workgroup, as
 //     [tap][octet][output channel] 16-byte rows, so a step's A operands are
 //     LDS reads (16 lanes read 16 consecutive rows);
+//   * the input tile is planar by octet, [octet][pixel]: the 16 lanes of a
+//     read take 16 consecutive pixels (256 contiguous bytes, no bank
+//     conflicts) and a tap's pixel offset is a constant, so with the steps
+//     unrolled every LDS address is a per-lane base plus an immediate;
 //   * a workgroup of 8 waves covers 8 rows x 64 columns, one row per wave
 //     (two waves per SIMD hide each other's LDS latency);
 //   * workgroups are persistent (one per CU: for 64 -> 64 the weights and
@@ -211,6 +215,16 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) void c
 //     the next tile's input into registers while computing the current one.
 constexpr int kWideRows = 8;
 constexpr int kWideTileRows = kWideRows + 2;
+constexpr int kWidePix = kWideTileRows * kTileCols;          // 660 pixels per tile
+
+typedef float f32x2 __attribute__((ext_vector_type(2)));
+typedef __bf16 bf16x2 __attribute__((ext_vector_type(2)));
+
+// two floats -> packed BF16 pair, round to nearest even (v_cvt_pk_bf16_f32)
+__device__ __forceinline__ uint32_t pack_bf16x2(float a, float b)
+{
+    return __builtin_bit_cast(uint32_t, __builtin_convertvector((f32x2){a, b}, bf16x2));
+}
 
 template <int K, int M, int MOUT, int SIGN, int POST>
 __global__ __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(2, 2))) void conv_wide_kernel(
@@ -220,7 +234,7 @@ __global__ __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(2, 2))) voi
     constexpr int KO = K / 8, MT = M / 16;
     constexpr int STEPS = (9 * KO + 3) / 4;
     constexpr bool TAP_UNIFORM = KO >= 4;                    // a step's 4 octets lie in one tap
-    constexpr int TILE = kWideTileRows * kTileCols * KO;
+    constexpr int TILE = KO * kWidePix;
     constexpr int WSLOTS = 9 * KO * M;
     constexpr int PER_THREAD = (TILE + 511) / 512;
     constexpr int WPER = (WSLOTS + 511) / 512;
@@ -259,9 +273,9 @@ __global__ __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(2, 2))) voi
         origin(b, n, r0, c0);
 #pragma unroll
         for (int j = 0; j < PER_THREAD; j++) {
-            const int idx = threadIdx.x + 512 * j;
-            const int q = idx % KO, rest = idx / KO;
-            const int gc = c0 - 1 + rest % kTileCols, gr = r0 - 1 + rest / kTileCols;
+            const int idx = threadIdx.x + 512 * j;          // over (pixel, octet), octet fastest
+            const int q = idx % KO, pix = idx / KO;
+            const int gc = c0 - 1 + pix % kTileCols, gr = r0 - 1 + pix / kTileCols;
             stage[j] = make_uint4(0, 0, 0, 0);
             if (idx < TILE && gr >= 0 && gr < H && gc >= 0 && gc < W)
                 stage[j] = *(const uint4*)(in + (((size_t)n * H + gr) * W + gc) * K + 8 * q);
@@ -271,8 +285,8 @@ __global__ __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(2, 2))) voi
 #pragma unroll
         for (int j = 0; j < PER_THREAD; j++) {
             const int idx = threadIdx.x + 512 * j;
-            const int q = idx % KO, rest = idx / KO;
-            if (idx < TILE) tile[rest * KO + swz<KO>(rest, q)] = stage[j];
+            const int q = idx % KO, pix = idx / KO;
+            if (idx < TILE) tile[q * kWidePix + pix] = stage[j];
         }
     };
 
@@ -280,6 +294,9 @@ __global__ __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(2, 2))) voi
     fetch(blk);
     put();
     __syncthreads();
+    // per-lane bases: this lane's pixel (row wv + 1, column 1 + li) and octet g
+    const uint4* tbase = tile + g * kWidePix + (wv + 1) * kTileCols + 1 + li;
+    const uint4* wbase = wl + g * M + li;
     for (;;) {
         const long long next = blk + gridDim.x;
         const bool more = next < nblk;                       // uniform over the workgroup
@@ -292,28 +309,32 @@ __global__ __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(2, 2))) voi
         for (int a = 0; a < MT; a++)
 #pragma unroll
             for (int b = 0; b < 4; b++) acc[a][b] = f32x4{0.f, 0.f, 0.f, 0.f};
-        // unrolled by 2 only: hoisting every step's LDS reads would exhaust
-        // registers
-#pragma unroll 2
-        for (int s = 0; s < STEPS; s++) {
-            int t, q;
-            if (TAP_UNIFORM) {
-                t = (4 * s) / KO;
-                q = (4 * s) % KO + g;
-            } else {
-                t = (4 * s + g) / KO;
-                q = (4 * s + g) % KO;
-            }
-            const bool tv = t < 9;
-            const int dy = SIGN * tap_dy(t), dx = SIGN * tap_dx(t);
-            bf16x8 A[MT];
 #pragma unroll
-            for (int a = 0; a < MT; a++)
-                A[a] = as_bf16x8(tv ? wl[(t * KO + q) * M + a * 16 + li] : make_uint4(0, 0, 0, 0));
+        for (int s = 0; s < STEPS; s++) {
+            // octet o = 4s + g: tap t = o / KO, octet q = o % KO; with
+            // TAP_UNIFORM, t and q - g are constants of the step
+            bf16x8 A[MT];
+            const int t = TAP_UNIFORM ? (4 * s) / KO : (4 * s + g) / KO;
+            const int q = TAP_UNIFORM ? (4 * s) % KO + g : (4 * s + g) % KO;
+            const bool tv = t < 9;
+            const int d = SIGN * (tap_dy(t) * kTileCols + tap_dx(t));   // tap's pixel offset
+#pragma unroll
+            for (int a = 0; a < MT; a++) {
+                uint4 v = make_uint4(0, 0, 0, 0);
+                if (TAP_UNIFORM)
+                    v = wbase[((4 * s) / KO * KO + (4 * s) % KO) * M + a * 16];
+                else if (tv)
+                    v = wl[(t * KO + q) * M + a * 16 + li];
+                A[a] = as_bf16x8(v);
+            }
 #pragma unroll
             for (int b = 0; b < 4; b++) {
-                const int p = (wv + 1 + dy) * kTileCols + 1 + dx + b * 16 + li;
-                const bf16x8 B = as_bf16x8(tv ? tile[p * KO + swz<KO>(p, q)] : make_uint4(0, 0, 0, 0));
+                uint4 v = make_uint4(0, 0, 0, 0);
+                if (TAP_UNIFORM)
+                    v = tbase[((4 * s) % KO) * kWidePix + d + b * 16];
+                else if (tv)
+                    v = tile[q * kWidePix + (wv + 1) * kTileCols + 1 + li + d + b * 16];
+                const bf16x8 B = as_bf16x8(v);
 #pragma unroll
                 for (int a = 0; a < MT; a++)
                     acc[a][b] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(A[a], B, acc[a][b], 0, 0, 0);
@@ -350,10 +371,8 @@ __global__ __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(2, 2))) voi
                         if ((av.y & 0x7FFFu) == 0) v[2] = 0.f;
                         if ((av.y & 0x7FFF0000u) == 0) v[3] = 0.f;
                     }
-                    uint2 st;
-                    st.x = (uint32_t)to_bf16(v[0]) | ((uint32_t)to_bf16(v[1]) << 16);
-                    st.y = (uint32_t)to_bf16(v[2]) | ((uint32_t)to_bf16(v[3]) << 16);
-                    *(uint2*)((uint16_t*)out + p * MOUT + m0) = st;
+                    *(uint2*)((uint16_t*)out + p * MOUT + m0) = make_uint2(pack_bf16x2(v[0], v[1]),
+                                                                           pack_bf16x2(v[2], v[3]));
                 }
             }
         }
