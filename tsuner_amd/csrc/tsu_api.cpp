@@ -115,6 +115,7 @@ void make_dev_params(const tsu_sim_params* p, const tsu_scheme* s, DevParams* P)
     P->sqrtFR = std::sqrt(p->FR);
     P->cbk = 1.64 + 0.4 * std::pow(p->DS / p->Dsg, 1.64);
     P->vs1p = P->vs * (1 - p->porosity);
+    P->inv_vs = 1.0 / P->vs;
     P->grass_k = s->grass_ag / (1.0 - p->porosity);
     P->manning_n = s->manning_n;
     P->bedload = s->bedload;

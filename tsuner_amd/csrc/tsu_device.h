@@ -57,6 +57,7 @@ struct DevParams {
     double cfl_half;        // CFL * 0.5            (Volume_kernel.cu:39)
     double FR, epsilon_h, Q, Opar, OC, sqrtOC, HQB, fiveDS;
     double vs;              // settling velocity    (Reconstructor.cu:76)
+    double inv_vs;          // correctly rounded 1 / vs (shared-denominator quotients)
     double ka;              // alpha1 * pow(Rp, alpha2)   (SedFlux.cuh:56-65)
     double sqrtFR;
     double cbk;             // 1.64 + 0.4*pow(DS/Dsg, 1.64) (SedFlux.cuh:83)
@@ -157,23 +158,44 @@ struct Vel {
 template <typename R>
 __device__ __forceinline__ R speed(const Vel<R>& u) { return rsqrt_(u.s); }
 
+// Quotients x / d that share one denominator d.  FP64: y = 1/d is one IEEE
+// division; each quotient is q = x*y corrected by one residual step,
+// q + (x - d*q)*y with both products fused.  With y the correctly rounded
+// reciprocal and q within an ulp of x/d, that is the correctly rounded x/d
+// (Markstein's theorem), i.e. the value of x / d, provided the residual
+// does not underflow (|x| above 2^-969; x == 0 gives a zero).  FP32: the
+// solver's single-instruction reciprocal, as rdiv.
+template <typename R>
+struct Den {
+    R d, y;
+};
+__device__ __forceinline__ Den<double> den(double d) { return {d, 1.0 / d}; }
+__device__ __forceinline__ Den<float> den(float d) { return {d, __builtin_amdgcn_rcpf(d)}; }
+__device__ __forceinline__ double qdiv(double x, const Den<double>& D)
+{
+    const double q = x * D.y;
+    return fma(fma(-D.d, q, x), D.y, q);
+}
+__device__ __forceinline__ float qdiv(float x, const Den<float>& D) { return x * D.y; }
+
 // SedFlux.cuh:26-49 in FP64.  Divisions with a zero numerator are skipped
-// (0/d is 0); the cap test uses s instead of sqrt(s) (see kVmax2Cut).
-__device__ __forceinline__ Vel<double> velocity64(double h, double qx, double qy)
+// (0/d is 0); the cap test uses s instead of sqrt(s) (see kVmax2Cut).  D,
+// when given, is den(h + DBL_EPSILON) computed by the caller.
+__device__ __forceinline__ Vel<double> velocity64(double h, double qx, double qy, const Den<double>* D = nullptr)
 {
     Vel<double> u;
     u.x = 0.0;
     u.y = 0.0;
-    if (h >= kEps) {
-        double d = h + kEpsilon;
-        if (qx != 0.0) u.x = qx / d;
-        if (qy != 0.0) u.y = qy / d;
+    if (h >= kEps && (qx != 0.0 || qy != 0.0)) {
+        const Den<double> Dh = D ? *D : den(h + kEpsilon);
+        if (qx != 0.0) u.x = qdiv(qx, Dh);
+        if (qy != 0.0) u.y = qdiv(qy, Dh);
     }
     double s = u.x * u.x + u.y * u.y;
     if (s > kVmax2Cut) {
-        double m = sqrt(s);
-        u.x = u.x / m * kVmax;
-        u.y = u.y / m * kVmax;
+        const Den<double> Dm = den(sqrt(s));
+        u.x = qdiv(u.x, Dm) * kVmax;
+        u.y = qdiv(u.y, Dm) * kVmax;
         s = u.x * u.x + u.y * u.y;
     }
     u.s = s;
@@ -208,17 +230,41 @@ __device__ __forceinline__ Vel<float> velocity(float h, float qx, float qy)
     return u;
 }
 
+// Velocity and concentration hc / (h + DBL_EPSILON) of one state
+// (SedFlux.cuh:26-49, 103-132); in FP64 the three quotients share one
+// reciprocal of h + DBL_EPSILON.
+template <bool SED>
+__device__ __forceinline__ void velocity_conc(double h, double qx, double qy, double hc, Vel<double>& u, double& c)
+{
+    c = 0.0;
+    const bool need_u = h >= kEps && (qx != 0.0 || qy != 0.0);
+    const bool need_c = SED && hc != 0.0;
+    if (need_u || need_c) {
+        const Den<double> D = den(h + kEpsilon);
+        if (need_c) c = qdiv(hc, D);
+        u = velocity64(h, qx, qy, &D);
+    } else {
+        u = velocity64(h, qx, qy);
+    }
+}
+
+template <bool SED>
+__device__ __forceinline__ void velocity_conc(float h, float qx, float qy, float hc, Vel<float>& u, float& c)
+{
+    c = 0.f;
+    if (SED && hc != 0.f) c = rdiv(hc, h + (float)kEpsilon);
+    u = velocity(h, qx, qy);
+}
+
 // SedFlux.cuh:118-132
 template <typename R, bool SED>
 __device__ __forceinline__ void state_filter(R& h, R& qx, R& qy, R& hc)
 {
     h = (h > (R)0) ? h : (R)0;
-    R c = (R)0;
-    if (SED) {
-        if (hc != (R)0) c = rdiv(hc, h + (R)kEpsilon);
-        c = (c > (R)0) ? c : (R)0;
-    }
-    Vel<R> u = velocity(h, qx, qy);
+    R c;
+    Vel<R> u;
+    velocity_conc<SED>(h, qx, qy, hc, u, c);
+    if (SED) c = (c > (R)0) ? c : (R)0;
     qx = h * u.x;
     qy = h * u.y;
     if (SED) hc = h * c;
@@ -286,13 +332,10 @@ __device__ __forceinline__ void riemann(const DevParams& P, bool bed_mobile,
         for (int f = 0; f < (SED ? 5 : 3); f++) Dm[f] = Dp[f] = zero;
         return;
     }
-    Vel<R> u0 = velocity(hl, qnl, qtl);
-    Vel<R> u1 = velocity(hr, qnr, qtr);
-    R c0 = zero, c1 = zero;
-    if (SED) {
-        if (hcl != zero) c0 = rdiv(hcl, hl + (R)kEpsilon);
-        if (hcr != zero) c1 = rdiv(hcr, hr + (R)kEpsilon);
-    }
+    Vel<R> u0, u1;
+    R c0, c1;
+    velocity_conc<SED>(hl, qnl, qtl, hcl, u0, c0);
+    velocity_conc<SED>(hr, qnr, qtr, hcr, u1, c1);
     R q0n = u0.x * h0, q0t = u0.y * h0;
     R q1n = u1.x * h1, q1t = u1.y * h1;
     R deta = h1 - h0;
@@ -326,8 +369,9 @@ __device__ __forceinline__ void riemann(const DevParams& P, bool bed_mobile,
     R dS = SR - SL;
     if (dS > (R)kEpsilon) {
         if constexpr (sizeof(R) == 8) {
-            a0 = (SR * fabs(SL) - SL * fabs(SR)) / dS;
-            a1 = (fabs(SR) - fabs(SL)) / dS;
+            const Den<double> D = den(dS);
+            a0 = qdiv(SR * fabs(SL) - SL * fabs(SR), D);
+            a1 = qdiv(fabs(SR) - fabs(SL), D);
         } else {
             R r = __builtin_amdgcn_rcpf(dS);
             a0 = (SR * rabs(SL) - SL * rabs(SR)) * r;
@@ -387,7 +431,7 @@ __device__ __forceinline__ R erosion_rate(const DevParams& P, R h, R modu, R hcv
 {
     R c = (R)0;
     if (hcv != (R)0) c = rdiv(hcv, h + (R)P.epsilon_h);
-    R Z = rdiv((R)P.ka * (modu * (R)P.sqrtFR), (R)P.vs);
+    R Z = qdiv((R)P.ka * (modu * (R)P.sqrtFR), Den<R>{(R)P.vs, (R)P.inv_vs});
     R Z5 = Z * Z * Z * Z * Z;
     R Es = rdiv((R)1.3e-7 * Z5, (R)1 + (R)4.3e-7 * Z5);
     R Fe = (R)P.vs1p * Es;

@@ -138,12 +138,11 @@ __device__ __forceinline__ void load_cell(const S* in, const S* Hin, long long n
 template <typename R, bool SED>
 __device__ __forceinline__ void derive_cell(Cell<R>& v)
 {
-    Vel<R> u = velocity(v.h, v.qx, v.qy);
+    Vel<R> u;
+    velocity_conc<SED>(v.h, v.qx, v.qy, v.hc, u, v.c);
     v.ux = u.x;
     v.uy = u.y;
     v.us = u.s;
-    v.c = (R)0;
-    if (SED && v.hc != (R)0) v.c = rdiv(v.hc, v.h + (R)kEpsilon);
 }
 
 // Reconstructor.cu:146-194 along one axis: faces on the minus side (Fm) and
@@ -445,12 +444,17 @@ __device__ __forceinline__ void stage_body(const StageArgs& a, const int tile)
                 k2 -= dtayG * R0.h * detay0;
             }
             Vel<R> un;
+            // the quadratic law divides by h^n + DBL_EPSILON, the velocity's
+            // denominator: one shared reciprocal (FP64)
+            const Den<R> Dh = den(v0[0] + (R)kEpsilon);
             if (a.k == 0) {   // U^n is this stage's input: its velocity is known
                 un.x = R0.ux;
                 un.y = R0.uy;
                 un.s = R0.us;
-            } else {
+            } else if constexpr (COMP) {
                 un = velocity(v0[0], v0[1], v0[2]);
+            } else {
+                un = velocity64(v0[0], v0[1], v0[2], &Dh);
             }
             const R modu = speed(un);
             R cf;
@@ -462,16 +466,12 @@ __device__ __forceinline__ void stage_body(const StageArgs& a, const int tile)
                 else
                     cf = dtGnn * modu / (hp * cbrt(hp));
             } else {
-                cf = rdiv(dtFR * modu, v0[0] + (R)kEpsilon);
+                cf = qdiv(dtFR * modu, Dh);
             }
-            if constexpr (COMP) {
-                R rc = __builtin_amdgcn_rcpf((R)1 + cf);
-                k1 *= rc;
-                k2 *= rc;
-            } else {
-                k1 /= (1.0 + cf);
-                k2 /= (1.0 + cf);
-            }
+            // Reconstructor.cu:52-53: both momenta divided by 1 + c
+            const Den<R> Dc = den((R)1 + cf);
+            k1 = qdiv(k1, Dc);
+            k2 = qdiv(k2, Dc);
             R phi = (R)0;
             if (SED && P.suspension) phi = erosion_rate(P, v0[0], modu, v0[4]);
 
