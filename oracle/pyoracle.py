@@ -96,6 +96,8 @@ def lib():
         L.orc_erosion_deposition.argtypes = [C.POINTER(OrcSimParams), C.POINTER(OrcScheme), C.c_double, C.c_double,
                                              C.c_double, C.c_double, _dp]
         L.orc_derive_params.argtypes = [C.POINTER(OrcSimParams)]
+        L.orc_shared_quotient_mismatches.restype = C.c_long
+        L.orc_shared_quotient_mismatches.argtypes = [C.c_long, C.c_ulonglong]
         _lib = L
     return _lib
 

@@ -106,6 +106,9 @@ void   orc_erosion_deposition(const orc_sim_params *p, const orc_scheme *s,
                               double h, double qx, double qy, double hc, double *phi);
 /* Derive Q, Opar, HQB and FR/8 from data.dat values (Problem_loader.cxx:337-342). */
 void   orc_derive_params(orc_sim_params *p);
+/* Test support (tsu_quotient.c): seeded check of the shared-denominator
+ * quotient used by the FP64 GPU path against x / d; returns mismatches. */
+long   orc_shared_quotient_mismatches(long n, unsigned long long seed);
 
 #ifdef __cplusplus
 }

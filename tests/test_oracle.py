@@ -425,3 +425,16 @@ def test_carrier_greenspan_runup_convergence():
     e4, r4 = _cg_run(4.0)
     assert e8 < 5e-3 and e4 < 0.6 * e8, (e8, e4)
     assert abs(r4) < 0.03 and abs(r4) < abs(r8), (r8, r4)
+
+
+def test_shared_denominator_quotient_is_the_ieee_quotient(oracle):
+    """The FP64 GPU path divides several numerators by one denominator with
+    one reciprocal and an FMA residual step (Markstein); on seeded random
+    pairs over the solver's ranges it equals x / d every time.  A variant
+    without the residual step (q = x * y alone) must fail the same check,
+    so the check can tell."""
+    assert oracle.lib().orc_shared_quotient_mismatches(2_000_000, 12345) == 0
+    rng = np.random.default_rng(1)
+    d = np.abs(rng.normal(size=100000)) + 1e-3
+    x = rng.normal(size=100000)
+    assert np.any(x * (1.0 / d) != x / d)
