@@ -102,3 +102,19 @@ def test_invalid_arguments_rejected_before_device_use(bad):
     rc = _lib.lib().tsu_state_create(C.byref(h), _lib.make_params(sc.params), _lib.make_scheme(s), prec, 0)
     assert rc in (3, 5)
     assert not h.value
+
+
+def test_network_kernel_kinds_validated_before_device_use():
+    """Unknown kinds (a weight-gradient kind given to the convolution, an
+    unknown flag) and misaligned operands are rejected before any device
+    work; the 64-wide weight-gradient kinds report their workspace."""
+    L = _lib.lib()
+    p = C.c_void_p(16)
+    for kind in (5, 5 | 16, 32, 1 | 32, -1):
+        assert L.tsu_nn_conv(kind, p, p, p, p, p, 1, 1, 1, None) == 3, kind
+    assert L.tsu_nn_conv(1, C.c_void_p(8), p, p, None, p, 1, 1, 1, None) == 3      # misaligned input
+    assert L.tsu_nn_conv(3 | 16, p, p, None, None, p, 1, 1, 1, None) == 3          # data gradient needs act
+    for kind, my, kx in ((5, 32, 8), (6, 32, 32), (7, 8, 32), (5 | 16, 64, 8), (6 | 16, 64, 64), (7 | 16, 8, 64)):
+        assert L.tsu_nn_wgrad_workspace(kind) == 9 * 256 * my * kx + 256 * my, kind
+    for kind in (0, 4, 8, 6 | 32):
+        assert L.tsu_nn_wgrad_workspace(kind) == -1, kind
