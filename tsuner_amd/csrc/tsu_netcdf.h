@@ -2,7 +2,7 @@
 // "CDF\002"), enough for the solver output file: fixed dimensions plus one
 // unlimited record dimension, double and text attributes, double variables.
 //
-// libnetcdf is not available to this build, so the format is written from
+// The library does not depend on libnetcdf: the format is written from
 // the published NetCDF classic file format specification (big-endian
 // header: magic, numrecs, dim_list, gatt_list, var_list; non-record
 // variables stored contiguously after the header; record variables
