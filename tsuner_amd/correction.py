@@ -195,10 +195,11 @@ def generate(seeds: Sequence[int], windows: int, scale_out: Sequence[float], pre
 SCALE_OUT_FLOOR = [1e-3, 1e-2, 1e-2, 1e-4]
 
 
-def normalise(ds: Dataset, chunk: int = 64) -> List[float]:
+def normalise(ds: Dataset, chunk: int = 64, weights: Sequence[float] = (1.0, 1.0, 1.0, 1.0)) -> List[float]:
     """Sets the output scales to the RMS of the raw targets over the whole
-    set (not below SCALE_OUT_FLOOR) and divides the targets by them, in
-    chunks through host memory.  Returns the scales."""
+    set (not below SCALE_OUT_FLOOR) divided by sqrt(weight), and divides the
+    targets by them, in chunks through host memory, so that channel c weighs
+    weights[c] in the mean-squared-error loss.  Returns the scales."""
     import torch
     n = ds.count
     ss = np.zeros(4)
@@ -207,7 +208,7 @@ def normalise(ds: Dataset, chunk: int = 64) -> List[float]:
         y = ds.y[i:min(n, i + chunk)].cpu().numpy().reshape(-1, 4).astype(np.float64)
         ss += (y * y).sum(axis=0)
         cnt += y.shape[0]
-    scale = [float(max(np.sqrt(s / max(cnt, 1)), f)) for s, f in zip(ss, SCALE_OUT_FLOOR)]
+    scale = [float(max(np.sqrt(s / max(cnt, 1)), f) / np.sqrt(w)) for s, f, w in zip(ss, SCALE_OUT_FLOOR, weights)]
     inv = (1.0 / np.asarray(scale, np.float64)).astype(np.float32)
     for i in range(0, n, chunk):
         y = ds.y[i:min(n, i + chunk)].cpu().numpy()
@@ -614,6 +615,7 @@ def main(argv=None):
     ap.add_argument("--rounds", type=int, default=0, help="on-trajectory data rounds after the first training")
     ap.add_argument("--batch", type=int, default=8)
     ap.add_argument("--width", type=int, default=32, choices=[32, 64], help="hidden channels")
+    ap.add_argument("--loss-weights", default="1,1,1,1", help="loss weights of the h, qx, qy, H corrections")
     ap.add_argument("--mids", type=int, default=2, help="hidden 3x3 layers between the input and output ones")
     ap.add_argument("--lr", type=float, default=2e-3)
     ap.add_argument("--heldout", type=int, default=200)
@@ -628,7 +630,7 @@ def main(argv=None):
     if a.cmd in ("train", "all"):
         t0 = time.perf_counter()
         ds = generate(list(range(1, a.seeds + 1)), a.windows, [1.0, 1.0, 1.0, 1.0])
-        scale_out = normalise(ds)
+        scale_out = normalise(ds, weights=[float(v) for v in a.loss_weights.split(",")])
         log("[train] output scales %s" % scale_out)
         t_data = time.perf_counter() - t0
         log("[train] %d pairs in %.1f s" % (ds.count, t_data))
@@ -648,6 +650,7 @@ def main(argv=None):
             hists.append(hist)
         t_train = time.perf_counter() - t1
         info = {"pairs": pairs, "train_seeds": ds.seeds, "windows": a.windows, "epochs": a.epochs,
+                "loss_weights": a.loss_weights,
                 "rounds": a.rounds, "batch": a.batch, "lr": a.lr, "loss": hists, "data_s": t_data,
                 "train_s": t_train}
         save(a.weights, net, scale_out, info)
