@@ -358,7 +358,7 @@ def test_carrier_greenspan_fp64_parity_and_fp32_accuracy(oracle):
 def test_runup_beach_512_at_its_configured_size():
     """The 512^2 runup-beach configuration (dx 20 m, Gaussian hump, walls,
     quadratic friction, no sediment) at full size: 100 steps in four chunks
-    (9 strips x 64 tiles, float-narrowed dt), bit-identical to the oracle;
+    (9 strips of 2-row tiles, float-narrowed dt), bit-identical to the oracle;
     and the sediment shelf at 256^2 with its N-wave inflow, 120 steps."""
     run_pair(S.plain_swe(n=512, dx=20.0), 100, chunks=4)
     run_pair(S.sediment_shelf(n=256, dx=10.0, seed=2), 120, chunks=2)
