@@ -85,9 +85,16 @@ def test_features_restrict_target_apply(precision):
     out = rng.normal(0, 1, size=(2, 16, 16, 4)).astype(np.float32)
     before = [coarse.state(i) for i in range(2)]
     tout = torch.from_numpy(out).cuda()
+    resid = torch.zeros(2, dtype=torch.float32, device="cuda")
     torch.cuda.synchronize()
-    coarse.nn_apply(SCALE_OUT, WET, tout.data_ptr(), 3)
+    coarse.nn_apply(SCALE_OUT, WET, tout.data_ptr(), 3, resid.data_ptr())
     coarse.synchronize()
+    # residual indicator: largest |correction of h - H| over each member's wet cells
+    got_r = resid.cpu().numpy()
+    for i in range(2):
+        wet = before[i][0] > WET
+        d = np.abs(out[i, ..., 0] * np.float32(SCALE_OUT[0]) - out[i, ..., 3] * np.float32(SCALE_OUT[3]))
+        np.testing.assert_allclose(got_r[i], d[wet].max(), rtol=1e-6)
     for i in range(2):
         B = before[i]
         was_wet = B[0] > WET

@@ -680,9 +680,17 @@ __global__ __launch_bounds__(256) void apply_kernel(const StageArgs* __restrict_
         }
     }
     if (resid) {
-        // non-negative floats order like their bit patterns
+        // one atomic per workgroup (the early return above is uniform over
+        // it: one member per grid row); non-negative floats order like their
+        // bit patterns
+        __shared__ float wmax[4];
         for (int off = 32; off > 0; off >>= 1) rmax = fmaxf(rmax, __shfl_xor(rmax, off));
-        if ((threadIdx.x & 63) == 0 && rmax > 0.f) atomicMax((unsigned int*)(resid + i), __float_as_uint(rmax));
+        if ((threadIdx.x & 63) == 0) wmax[threadIdx.x >> 6] = rmax;
+        __syncthreads();
+        if (threadIdx.x == 0) {
+            const float m = fmaxf(fmaxf(wmax[0], wmax[1]), fmaxf(wmax[2], wmax[3]));
+            if (m > 0.f) atomicMax((unsigned int*)(resid + i), __float_as_uint(m));
+        }
     }
 }
 
