@@ -190,7 +190,8 @@ def run_correction(heldout, fine_steps):
     ev = K.evaluate(net, scale_out, list(range(300001, 300001 + heldout)), fine_steps=fine_steps)
     keys = ("peak_eta_err_mean_m", "frac_peak_eta_within_5cm", "peak_eta_sea_err_mean_m",
             "frac_peak_eta_sea_within_5cm", "nearshore_mean_abs_eta_err_m", "runup_rel_err_mean",
-            "frac_runup_within_10pct", "sims_per_hr", "speedup_vs_fine", "fallback_frac")
+            "frac_runup_within_10pct", "bed_change_mae_mean_m", "bed_change_fine_mean_abs_m", "sims_per_hr",
+            "speedup_vs_fine", "fallback_frac")
 
     def pick(d):
         return {k: d[k] for k in keys if k in d}
