@@ -1,6 +1,8 @@
 """The learned-correction pipeline end to end at a small size (fine 64^2,
 coarse 16^2): training pairs line up in time, a zero network leaves the
-coarse run unchanged bit for bit, and training on the pairs lowers the loss."""
+coarse run unchanged bit for bit, and training on the pairs lowers the loss;
+and at full size, what the shipped network gains over the coarse solver
+alone."""
 import numpy as np
 import pytest
 
