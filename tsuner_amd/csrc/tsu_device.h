@@ -452,6 +452,9 @@ __device__ __forceinline__ R cell_dt(const DevParams& P, R h, R qx, R qy)
     R cg = rsqrt_((R)kG * h);
     R lx = rabs(u.x) + cg;
     R ly = rabs(u.y) + cg;
+    // square cells: rounding is monotone, so min(dx/lx, dx/ly) is dx/max(lx, ly)
+    // (lx, ly >= 0, never NaN; a dry cell gives dx/0 = inf either way)
+    if (P.dx == P.dy) return (R)P.cfl_half * rdiv((R)P.dx, rmax(lx, ly));
     return (R)P.cfl_half * rmin(rdiv((R)P.dx, lx), rdiv((R)P.dy, ly));
 }
 
