@@ -116,7 +116,9 @@ __device__ __forceinline__ int sgn(R x)
 
 // Reconstructor.cu:19-31.  When a and b do not share a strict sign the
 // numerator a|b| + b|a| is exactly zero (for finite products), so the
-// division is only evaluated for same-sign slopes.
+// division is only evaluated for same-sign slopes; there a|b| and b|a| are
+// both +-RN(ab) (rounding is sign-symmetric), so the numerator is
+// +-2 RN(ab) exactly.
 template <typename R>
 __device__ __forceinline__ R lim_vanleer(R a, R b)
 {
@@ -124,7 +126,8 @@ __device__ __forceinline__ R lim_vanleer(R a, R b)
     R s = aa + bb;
     R d = (R)0;
     if ((a > (R)0 && b > (R)0) || (a < (R)0 && b < (R)0)) {
-        if (s > (R)kEpsilon) d = rdiv(a * bb + b * aa, s);
+        const R p = a * b;                  // > 0 for same-sign slopes
+        if (s > (R)kEpsilon) d = rdiv(a > (R)0 ? p + p : -(p + p), s);
     }
     return d;
 }
