@@ -292,7 +292,9 @@ TSU_API int tsu_kat_edge_flux(const tsu_sim_params *params, const tsu_scheme *sc
                               const int *fixed, double *Dm, double *Dp, int device);
 /* what: 0 limiter(a,b) -> o1; 1 velocity(h=a,qx=b,qy=c) -> o1,o2,o3=|u|;
  * 2 state filter(a,b,c,d) -> o1..o4; 3 cell dt(a,b,c) -> o1;
- * 4 erosion rate(h=a,qx=b,qy=c,hc=d) -> o1; 5 bedload(h=a,un=b,ut=c) -> o1. */
+ * 4 erosion rate(h=a,qx=b,qy=c,hc=d) -> o1; 5 bedload(h=a,un=b,ut=c) -> o1;
+ * 6 shared-denominator division: o1 = 1/a, o2 = b/a (a in [2^-60, 2^60]),
+ *   o3, o4 the same by IEEE division. */
 TSU_API int tsu_kat_point(const tsu_sim_params *params, const tsu_scheme *scheme, int what, int n,
                           const double *a, const double *b, const double *c, const double *d,
                           double *o1, double *o2, double *o3, double *o4, int device);

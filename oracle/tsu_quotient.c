@@ -8,7 +8,9 @@
  * numerator x, q = x*y followed by q + (x - d*q)*y with both products fused.
  * Markstein's theorem says this is the correctly rounded x/d when y is the
  * correctly rounded reciprocal, q is within an ulp and the residual does not
- * underflow.  orc_shared_quotient_mismatches() counts, over n seeded random
+ * underflow.  (The device computes y itself from the hardware reciprocal
+ * estimate; that y equals 1.0 / d, all-ones significands included, is
+ * checked on the device against numpy.)  orc_shared_quotient_mismatches() counts, over n seeded random
  * pairs spread over the ranges the solver meets (and near-exact quotients),
  * the pairs where that value differs from x / d.
  */
@@ -55,3 +57,4 @@ long orc_shared_quotient_mismatches(long n, unsigned long long seed)
     }
     return bad;
 }
+

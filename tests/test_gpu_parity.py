@@ -362,3 +362,30 @@ def test_runup_beach_512_at_its_configured_size():
     and the sediment shelf at 256^2 with its N-wave inflow, 120 steps."""
     run_pair(S.plain_swe(n=512, dx=20.0), 100, chunks=4)
     run_pair(S.sediment_shelf(n=256, dx=10.0, seed=2), 120, chunks=2)
+
+
+def test_shared_denominator_division_on_the_device():
+    """The device's reciprocal (hardware estimate, two Newton refinements, one
+    residual correction, no operand scaling) and shared-denominator quotient
+    equal numpy's correctly rounded 1/a and b/a over [2^-60, 2^60]: random
+    denominators, denominators with an all-ones significand (where a
+    reciprocal refinement can land on a rounding tie), h + DBL_EPSILON forms
+    and 1 + c forms."""
+    rng = np.random.default_rng(11)
+    sc = S.sediment_shelf(n=8, dx=10.0, inflow=False)
+    n = 200000
+    e = rng.integers(-60, 60, n)
+    a = np.ldexp(1.0 + rng.random(n), e)
+    ones = np.arange(n) % 4 == 0
+    a[ones] = np.ldexp(2.0 - 2.0 ** -52, e[ones] - 1)            # significand all ones
+    k = np.arange(n) % 4 == 1
+    a[k] = np.abs(rng.normal(size=k.sum())) * 10.0 ** rng.integers(-12, 4, k.sum()) + np.finfo(float).eps
+    k = np.arange(n) % 4 == 2
+    a[k] = 1.0 + np.abs(rng.normal(size=k.sum())) * 10.0 ** rng.integers(-16, 12, k.sum())
+    b = rng.normal(size=n) * 10.0 ** rng.integers(-30, 30, n)
+    y, q, yi, qi = kat_point(sc.params, sc.scheme, 6, a, b)
+    np.testing.assert_array_equal(yi, 1.0 / a)                  # the device's IEEE division
+    np.testing.assert_array_equal(qi, b / a)
+    bad = np.flatnonzero(y != 1.0 / a)
+    assert bad.size == 0, "1/a differs at %d inputs, e.g. a=%r: %r vs %r" % (bad.size, a[bad[0]], y[bad[0]], 1.0 / a[bad[0]])
+    np.testing.assert_array_equal(q, b / a)

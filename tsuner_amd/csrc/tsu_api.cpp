@@ -790,7 +790,7 @@ TSU_API int tsu_kat_point(const tsu_sim_params* p, const tsu_scheme* s, int what
                           const double* b, const double* c, const double* d, double* o1, double* o2, double* o3,
                           double* o4, int device)
 {
-    if (!p || !s || n < 0 || what < 0 || what > 5) return fail(TSU_ERR_ARG, "bad argument");
+    if (!p || !s || n < 0 || what < 0 || what > 6) return fail(TSU_ERR_ARG, "bad argument");
     if (n == 0) return TSU_OK;
     DevParams P;
     tsu::make_dev_params(p, s, &P);

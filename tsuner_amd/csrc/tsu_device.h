@@ -172,7 +172,23 @@ template <typename R>
 struct Den {
     R d, y;
 };
-__device__ __forceinline__ Den<double> den(double d) { return {d, 1.0 / d}; }
+
+// RN(1/d) for d in [2^-60, 2^60]: the steps of the IEEE f64 division with
+// numerator 1 (hardware reciprocal, two Newton refinements, one residual
+// correction) without its operand scaling and special-case fixup, which
+// leave operands and result unchanged in that range.
+__device__ __forceinline__ double recip_normal(double d)
+{
+    const double r0 = __builtin_amdgcn_rcp(d);
+    double r = fma(r0, fma(-d, r0, 1.0), r0);
+    r = fma(r, fma(-d, r, 1.0), r);
+    return fma(fma(-d, r, 1.0), r, r);
+}
+
+// Every shared denominator of the solver lies in [2^-60, 2^60]: h + DBL_EPSILON
+// (>= 2^-52), S_R - S_L (> DBL_EPSILON, a few hundred m/s at most), 1 + c
+// (>= 1, c <= dt FR 10 / DBL_EPSILON) and |u| > 10 under the speed cap.
+__device__ __forceinline__ Den<double> den(double d) { return {d, recip_normal(d)}; }
 __device__ __forceinline__ Den<float> den(float d) { return {d, __builtin_amdgcn_rcpf(d)}; }
 __device__ __forceinline__ double qdiv(double x, const Den<double>& D)
 {

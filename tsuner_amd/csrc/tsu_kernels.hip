@@ -835,6 +835,16 @@ __global__ void kat_point_kernel(int what, int law, int n, const double* a, cons
         o1[i] = bedload(P, a[i], u);
         break;
     }
+    case 6: {
+        // the shared-denominator division: 1 / a and b / a through den(a),
+        // beside the IEEE divisions
+        const Den<double> D = den(a[i]);
+        o1[i] = D.y;
+        o2[i] = qdiv(b[i], D);
+        o3[i] = 1.0 / a[i];
+        o4[i] = b[i] / a[i];
+        break;
+    }
     }
 }
 
