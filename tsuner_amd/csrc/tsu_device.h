@@ -101,6 +101,10 @@ __device__ __forceinline__ double rdiv(double a, double b) { return a / b; }
 __device__ __forceinline__ float rdiv(float a, float b) { return a * __builtin_amdgcn_rcpf(b); }
 __device__ __forceinline__ double rsqrt_(double x) { return sqrt(x); }
 __device__ __forceinline__ float rsqrt_(float x) { return __builtin_amdgcn_sqrtf(x); }
+// a*b + c: FP64 rounds the product and the sum separately, as the
+// reference (built without FMA contraction); the FP32 solver fuses them.
+__device__ __forceinline__ double rmad(double a, double b, double c) { return a * b + c; }
+__device__ __forceinline__ float rmad(float a, float b, float c) { return __builtin_fmaf(a, b, c); }
 __device__ __forceinline__ double rmin(double a, double b) { return fmin(a, b); }
 __device__ __forceinline__ float rmin(float a, float b) { return fminf(a, b); }
 __device__ __forceinline__ double rmax(double a, double b) { return fmax(a, b); }
@@ -402,20 +406,20 @@ __device__ __forceinline__ void riemann(const DevParams& P, bool bed_mobile,
     }
 
     R df0 = F10 - F00, df1 = F11 - F01, df2 = F12 - F02;
-    df1 += g * hm * deta;
+    df1 = rmad(g * hm, deta, df1);
     R DU0 = h1 - h0, DU1 = q1n - q0n, DU2 = q1t - q0t;
-    R v0 = a0 * DU0 + a1 * df0;
-    R v1 = a0 * DU1 + a1 * df1;
-    R v2 = a0 * DU2 + a1 * df2;
-    Dm[0] = half * (df0 - v0) + F00;
-    Dm[1] = half * (df1 - v1) + F01;
-    Dm[2] = half * (df2 - v2) + F02;
+    R v0 = rmad(a0, DU0, a1 * df0);
+    R v1 = rmad(a0, DU1, a1 * df1);
+    R v2 = rmad(a0, DU2, a1 * df2);
+    Dm[0] = rmad(half, df0 - v0, F00);
+    Dm[1] = rmad(half, df1 - v1, F01);
+    Dm[2] = rmad(half, df2 - v2, F02);
     if constexpr (sizeof(R) == 8)
         Dp[0] = half * (df0 + v0) - F10;
     else
         Dp[0] = -Dm[0];
-    Dp[1] = half * (df1 + v1) - F11;
-    Dp[2] = half * (df2 + v2) - F12;
+    Dp[1] = rmad(half, df1 + v1, -F11);
+    Dp[2] = rmad(half, df2 + v2, -F12);
 
     if (SED) {
         R hc0 = h0 * c0, hc1 = h1 * c1;
@@ -430,9 +434,9 @@ __device__ __forceinline__ void riemann(const DevParams& P, bool bed_mobile,
         int s0 = sgn(zr - zl), s1 = sgn(df3), s2 = sgn(a1);
         s1 = (s0 != 0) ? s1 * s0 : 0;
         R v3 = (s2 == s1) ? a1 * df3 : -a1 * df3;
-        R v4 = a0 * DU4 + a1 * df4;
-        Dm[3] = half * (df3 - v3) + F03;
-        Dm[4] = half * (df4 - v4) + F04;
+        R v4 = rmad(a0, DU4, a1 * df4);
+        Dm[3] = rmad(half, df3 - v3, F03);
+        Dm[4] = rmad(half, df4 - v4, F04);
         if constexpr (sizeof(R) == 8) {
             Dp[3] = half * (df3 + v3) - F13;
             Dp[4] = half * (df4 + v4) - F14;

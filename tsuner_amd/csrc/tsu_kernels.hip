@@ -171,19 +171,19 @@ __device__ __forceinline__ R recons_axis(int lim, const Cell<R>& ci, R hl, R Hl,
     Fp[3] = Hi + half * dz;
 
     R du = limiter(lim, ci.ux - ulx, urx - ci.ux);
-    R dq = hi * du + ci.ux * dh;
-    Fm[1] = ci.qx - half * dq;
-    Fp[1] = ci.qx + half * dq;
+    R dq = rmad(hi, du, ci.ux * dh);
+    Fm[1] = rmad(-half, dq, ci.qx);
+    Fp[1] = rmad(half, dq, ci.qx);
 
     du = limiter(lim, ci.uy - uly, ury - ci.uy);
-    dq = hi * du + ci.uy * dh;
-    Fm[2] = ci.qy - half * dq;
-    Fp[2] = ci.qy + half * dq;
+    dq = rmad(hi, du, ci.uy * dh);
+    Fm[2] = rmad(-half, dq, ci.qy);
+    Fp[2] = rmad(half, dq, ci.qy);
 
     if (SED) {
         du = limiter(lim, ci.c - cl, cr - ci.c);
-        Fm[4] = (ci.c - half * du) * Fm[0];
-        Fp[4] = (ci.c + half * du) * Fp[0];
+        Fm[4] = rmad(-half, du, ci.c) * Fm[0];
+        Fp[4] = rmad(half, du, ci.c) * Fp[0];
     } else {
         Fm[4] = Fp[4] = zero;
     }
