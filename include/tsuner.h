@@ -127,6 +127,33 @@ TSU_API int tsu_write_output(const char *output_prefix, const tsu_sim_params *pa
 TSU_API int tsu_parse_data_dat(const char *path, tsu_sim_params *params, char *prefix, int prefix_len,
                                char *files, int files_len, double *sea_level_correction);
 
+/* ---- real-data input (the reference's Read_files = 1 build) ------------- */
+
+/* Number of time levels of the free-surface series file (the length of its
+ * time/TIME dimension, Problem_loader.cxx:153-161): sizes CC and tCC for
+ * tsu_load_input_files.  NetCDF classic and NetCDF-4 (HDF5) files are read;
+ * the latter through the HDF5 C library loaded at run time (libhdf5.so, or
+ * the path in TSU_HDF5_LIB). */
+TSU_API int tsu_input_time_levels(const char *eta_file, unsigned long *time_levels);
+
+/* Replaces LoadProblemData's file branch (Problem_loader.cxx:355-624, with
+ * interpolations.cxx:46-95,368-1114): the bathymetry grid (variable
+ * "bathy", coordinates lon/x/LON and lat/y/LAT, depth positive down), the
+ * free-surface series eta_file (ha_lc/HA/ha, cm) with its siblings whose
+ * 4th character from the end is 'u' and 'v' (ua_lc/UA/ua, va_lc/VA/va,
+ * cm/s) and an optional fixed-bed map (NULL, "" or "0" for none; 1 = bed may
+ * move) become, on params->npx x params->npy cells, the Cartesian initial
+ * state h, hu (= 0), H, fixH and the boundary series CC
+ * ((2*npx+2*npy)*4*time_levels doubles, VolumeSolver.cu:172-202 layout) and
+ * tCC (time_levels doubles).  All buffers are caller-owned.  On return
+ * params holds xmin = ymin = 0, xmax, ymax in metres, dx, dy and nvol; its
+ * other fields are untouched.  Errors: TSU_ERR_ARG for shapes and sizes,
+ * TSU_ERR_HOST for unreadable files (tsu_last_error() names the file). */
+TSU_API int tsu_load_input_files(const char *bathy_file, const char *eta_file, const char *fixed_bed_file,
+                                 double sea_level_correction, tsu_sim_params *params, double *dVh,
+                                 tsu_double2 *dVhu, double *dVH, int *fixH, double *CC, double *tCC,
+                                 unsigned long time_levels);
+
 /* ---- handle API --------------------------------------------------------- */
 
 typedef struct tsu_state tsu_state;

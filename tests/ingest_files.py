@@ -1,0 +1,79 @@
+"""Synthetic input files of the real-data path, written with
+scipy.io.netcdf_file (NetCDF classic): a bathymetry grid and an NCTR-style
+<stem>h/u/v series in longitude/latitude, with the reference's variable
+names (Problem_loader.cxx:57-175, interpolations.cxx:386-460)."""
+from __future__ import annotations
+
+import os
+
+import numpy as np
+
+
+def write_bathy(path, lon, lat, bathy, var_dtype="f4", coord_names=("lon", "lat"), version=1):
+    from scipy.io import netcdf_file
+    with netcdf_file(str(path), "w", version=version) as f:
+        f.createDimension(coord_names[0], len(lon))
+        f.createDimension(coord_names[1], len(lat))
+        x = f.createVariable(coord_names[0], "f8", (coord_names[0],))
+        y = f.createVariable(coord_names[1], "f8", (coord_names[1],))
+        x[:] = lon
+        y[:] = lat
+        b = f.createVariable("bathy", var_dtype, (coord_names[1], coord_names[0]))
+        b[:] = bathy
+        b.units = "meters"
+        b.positive = "down"
+
+
+def write_series(stem, lon, lat, times, h, u, v, upper=True, version=1):
+    """stem + 'h.nc', 'u.nc', 'v.nc'; h, u, v (T, nlat, nlon) in cm and cm/s.
+    upper=True uses the NCTR names (HA, LON, LAT, TIME, unlimited time),
+    False the lower-case ones (ha_lc, lon, lat, time, fixed time)."""
+    from scipy.io import netcdf_file
+    names = dict(lon="LON", lat="LAT", time="TIME", h="HA", u="UA", v="VA") if upper else \
+        dict(lon="lon", lat="lat", time="time", h="ha_lc", u="ua_lc", v="va_lc")
+    paths = []
+    for c, data in (("h", h), ("u", u), ("v", v)):
+        path = str(stem) + c + ".nc"
+        with netcdf_file(path, "w", version=version) as f:
+            f.createDimension(names["time"], None if upper else len(times))
+            f.createDimension(names["lat"], len(lat))
+            f.createDimension(names["lon"], len(lon))
+            t = f.createVariable(names["time"], "f4", (names["time"],))
+            x = f.createVariable(names["lon"], "f4", (names["lon"],))
+            y = f.createVariable(names["lat"], "f4", (names["lat"],))
+            x[:] = lon
+            y[:] = lat
+            d = f.createVariable(names[c], "f4", (names["time"], names["lat"], names["lon"]))
+            d[:] = data
+            t[:] = times
+        paths.append(path)
+    return paths
+
+
+def coastal_case(tmp, nlon=90, nlat=70, T=8, seed=0, fill=False, upper=True):
+    """A sloping coast near 46.9 N: depth 40 m offshore (west) to 8 m of land
+    (east) with seeded bumps; a free-surface series of a long wave entering
+    from the west, sampled on a coarser forcing grid covering the bathymetry
+    with a margin.  Returns (bathy path, h-series path)."""
+    rng = np.random.default_rng(seed)
+    lon = 235.80 + np.arange(nlon) * (0.10 / (nlon - 1))
+    lat = 46.85 + np.arange(nlat) * (0.08 / (nlat - 1))
+    X, Y = np.meshgrid((lon - lon[0]) / (lon[-1] - lon[0]), (lat - lat[0]) / (lat[-1] - lat[0]))
+    bathy = 40.0 - 48.0 * X + 2.0 * np.sin(7 * Y + 3 * X) + rng.normal(0, 0.3, X.shape)
+    bpath = os.path.join(str(tmp), "coast_bathy.nc")
+    write_bathy(bpath, lon, lat, bathy.astype(np.float32))
+    flon = 235.70 + np.arange(48) * (0.30 / 47)
+    flat = 46.80 + np.arange(40) * (0.18 / 39)
+    times = 30.0 * (1 + np.arange(T))
+    FX, FY = np.meshgrid(flon, flat)
+    k = 2 * np.pi / 0.12
+    h = np.stack([60.0 * np.sin(k * (FX - 235.7) - 0.02 * t) * np.exp(-((FY - 46.9) / 0.2) ** 2) for t in times])
+    u = np.stack([25.0 * np.sin(k * (FX - 235.7) - 0.02 * t) for t in times])
+    v = np.stack([4.0 * np.cos(k * (FY - 46.8) - 0.02 * t) + 0 * FX for t in times])
+    if fill:
+        h[:, :3, :3] = 1e35
+        u[:, -2:, :] = -1e35
+    stem = os.path.join(str(tmp), "coast_runup_")
+    write_series(stem, flon, flat, times, h.astype(np.float32), u.astype(np.float32), v.astype(np.float32),
+                 upper=upper)
+    return bpath, stem + "h.nc"

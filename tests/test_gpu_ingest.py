@@ -1,0 +1,97 @@
+"""Real-data input end to end on the GPU: longitude/latitude input files
+(written here as NetCDF classic) -> tsu_load_input_files -> the solver with
+the loaded boundary series on all sides (VolumeSolver.cu:172-215), against
+the oracle running the oracle-loaded case; FP64 results value-equal.  Then
+the `tsuner data.dat` program on a data.dat naming such files."""
+import os
+import subprocess
+import sys
+
+import numpy as np
+import pytest
+
+import pyingest as OI
+from tsuner_amd import ingest
+from tsuner_amd import scenarios as S
+from tsuner_amd.solver import Solver
+
+from ingest_files import coastal_case
+
+pytestmark = pytest.mark.gpu
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+CLI = os.path.join(os.path.dirname(HERE), "tsuner_amd", "tsuner")
+EPS = np.finfo(float).eps
+
+
+def say(msg):
+    print(msg, flush=True)
+    sys.stdout.flush()
+
+
+def oracle_case(b, e, npx, npy, slr, scheme):
+    d = OI.load_problem_files(b, e, "0", slr, npx, npy)
+    p = dict(S.base_params(npx, npy, 1.0, 1.0))
+    for k in ("xmin", "xmax", "ymin", "ymax", "dx", "dy"):
+        p[k] = d[k]
+    V = S.pack_state(d["h"], 0 * d["h"], 0 * d["h"], d["H"])
+    return p, V, d["fixH"], d["CC"], d["tCC"]
+
+
+@pytest.mark.parametrize("npx,npy", [(96, 80), (131, 67)])
+def test_loaded_case_matches_oracle(tmp_path, oracle, npx, npy):
+    b, e = coastal_case(tmp_path, T=8, seed=npx)
+    sc = ingest.scenario_from_files(b, e, S.base_params(npx, npy, 1.0, 1.0), sea_level_correction=0.1)
+    p, V, fix, CC, tCC = oracle_case(b, e, npx, npy, 0.1, sc.scheme)
+    np.testing.assert_array_equal(sc.V, V)
+    np.testing.assert_array_equal(sc.CC, CC)
+    say("ingest case %dx%d, dx %.3f dy %.3f, %d time levels" % (npx, npy, p["dx"], p["dy"], len(tCC)))
+    o = oracle.OracleSim(p, sc.scheme, V, fix, CC, tCC)
+    g = Solver.from_scenario(sc)
+    assert g.dt == o.dt
+    for c in range(4):   # t passes tCC[T-2] = 210 s: inflow, then walls
+        o.step(60)
+        g.step(60)
+        a, w = o.state(), g.state()
+        assert np.isfinite(a).all()
+        np.testing.assert_array_equal(a, w, err_msg="after %d steps" % (60 * (c + 1)))
+        t, dt, _ = g.time()
+        assert t == o.t and dt == o.dt
+    assert o.t > tCC[-2]
+
+
+def test_cli_reads_input_files(tmp_path, oracle):
+    b, e = coastal_case(tmp_path, T=6, seed=4)
+    npx, npy, total, save = 48, 40, 120.0, 40.0
+    lines = ["# data.dat with input files", "# xmin", "0", "# xmax", "1", "# ymin", "0", "# ymax", "1",
+             "# total time", str(total), "# save time", str(save), "# npx", str(npx), "# npy", str(npy),
+             "# CFL", "0.9", "# epsilon_h", "1e-4", "# R", "2.65", "# DS", "2e-4", "# friction", "2.5e-2",
+             "# porosity", "0.4", "# critical theta", "0.0047", "# Dsg", "100e-6", "# output", "coast_run",
+             "# bathymetry", os.path.basename(b), "# eta", os.path.basename(e), "# fixed bed", "0",
+             "# sea level", "0.2", "# tides", "0"]
+    (tmp_path / "data.dat").write_text("\n".join(lines) + "\n")
+    say("cli with input files %dx%d" % (npx, npy))
+    r = subprocess.run([CLI, "data.dat", "0"], cwd=tmp_path, capture_output=True, text=True, timeout=120)
+    print(r.stdout[-1500:], r.stderr[-1500:], flush=True)
+    assert r.returncode == 0 and "Normal End." in r.stdout
+    scheme = S.default_scheme(extrema=1, bc=[S.BC_INFLOW] * 4)
+    p, V, fix, CC, tCC = oracle_case(b, e, npx, npy, 0.2, scheme)
+    p["total_time"], p["save_time"] = total, save
+    o = oracle.OracleSim(p, scheme, V, fix, CC, tCC)
+    recs, times = [], []
+    nxt = 0.0
+    while o.t < total:   # GPU/main_TSUNER.cu:157-209 save cadence
+        if o.t >= nxt:
+            recs.append(o.state())
+            times.append(o.t)
+            nxt += save
+        o.step(1)
+    recs.append(o.state())
+    times.append(o.t)
+    from scipy.io import netcdf_file
+    with netcdf_file(str(tmp_path / "coast_run.nc"), "r", mmap=False) as f:
+        np.testing.assert_array_equal(f.variables["time"][:], times)
+        for k, W in enumerate(recs):
+            np.testing.assert_array_equal(f.variables["eta"][k], W[0] - W[3])
+            np.testing.assert_array_equal(f.variables["ux"][k], W[1] / (W[0] + EPS))
+            np.testing.assert_array_equal(f.variables["b"][k], -W[3])

@@ -137,6 +137,9 @@ _PROTOS = {
     "tsu_bandwidth_probe": (C.c_int, [C.c_int, C.c_longlong, C.c_int, _dp]),
     "tsu_parse_data_dat": (C.c_int, [C.c_char_p, C.POINTER(SimParams), C.c_char_p, C.c_int, C.c_char_p, C.c_int,
                                      _dp]),
+    "tsu_input_time_levels": (C.c_int, [C.c_char_p, C.POINTER(C.c_ulong)]),
+    "tsu_load_input_files": (C.c_int, [C.c_char_p, C.c_char_p, C.c_char_p, C.c_double, C.POINTER(SimParams), _dp,
+                                       C.c_void_p, _dp, _ip, _dp, _dp, C.c_ulong]),
     "tsu_last_error": (C.c_char_p, []),
     "tsu_version": (C.c_char_p, []),
     "tsu_kat_edge_flux": (C.c_int, [C.POINTER(SimParams), C.POINTER(Scheme), C.c_int, _dp, _dp, _ip, _ip, _dp,

@@ -20,7 +20,7 @@ LIB = os.path.join(HERE, "libtsuner.so")
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 
 SOURCES = ["tsu_kernels.hip", "tsu_api.cpp", "tsu_run.cpp", "tsu_netcdf.cpp", "tsu_ensemble.cpp", "tsu_datadat.cpp",
-           "tsu_nn.hip", "tsu_nn_api.cpp"]
+           "tsu_nn.hip", "tsu_nn_api.cpp", "tsu_gridfile.cpp", "tsu_ingest.cpp"]
 CLI = os.path.join(HERE, "tsuner")
 
 COMMON = ["-O3", "-fPIC", "-std=c++17", "-ffp-contract=off", "-fno-fast-math", "-fvisibility=hidden",
@@ -46,7 +46,7 @@ def build_variant(out: str, defines) -> str:
     os.makedirs(bdir, exist_ok=True)
     with cf.ThreadPoolExecutor(max_workers=4) as ex:
         objs = list(ex.map(lambda s: _compile(s, defines, bdir), SOURCES))
-    cmd = [HIPCC, "--offload-arch=gfx950", "-shared", "-fPIC", "-o", out] + objs
+    cmd = [HIPCC, "--offload-arch=gfx950", "-shared", "-fPIC", "-o", out] + objs + ["-ldl"]
     r = subprocess.run(cmd, capture_output=True, text=True)
     if r.returncode != 0:
         raise RuntimeError("link failed: %s\n%s\n%s" % (" ".join(cmd), r.stdout, r.stderr))
@@ -58,7 +58,7 @@ def build(verbose: bool = False) -> str:
     with cf.ThreadPoolExecutor(max_workers=4) as ex:
         objs = list(ex.map(_compile, SOURCES))
     tmp = LIB + ".tmp"
-    cmd = [HIPCC, "--offload-arch=gfx950", "-shared", "-fPIC", "-o", tmp] + objs
+    cmd = [HIPCC, "--offload-arch=gfx950", "-shared", "-fPIC", "-o", tmp] + objs + ["-ldl"]
     r = subprocess.run(cmd, capture_output=True, text=True)
     if r.returncode != 0:
         raise RuntimeError("link failed: %s\n%s\n%s" % (" ".join(cmd), r.stdout, r.stderr))

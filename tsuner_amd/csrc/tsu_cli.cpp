@@ -5,10 +5,12 @@
 // Initial state, in order of precedence:
 //   --state FILE   raw little-endian binary: 5*npx*npy doubles (h, qx, qy,
 //                  H, hc planes) followed by npx*npy int32 fixed-bed flags;
+//   input files    when the bathymetry file named in data.dat exists: the
+//                  reference's Read_files = 1 build (Problem_loader.cxx:355-624)
+//                  through tsu_load_input_files, with the boundary series
+//                  CC/tCC passed on to TSUNER();
 //   otherwise      the reference's analytic example case (cini.cxx:22-61),
 //                  sampled at cell centres as Problem_loader.cxx:650-685 does.
-// NetCDF input files named in data.dat (the reference's Read_files = 1
-// path) are not read by this program.
 #include <sys/stat.h>
 
 #include <cmath>
@@ -90,7 +92,39 @@ int main(int argc, char** argv)
     std::vector<double> h(nv), H(nv);
     std::vector<tsu_double2> q(nv);
     std::vector<int> fix(nv, 1);
-    if (!state_file.empty()) {
+    std::vector<double> CC, tCC;
+    std::vector<std::string> names;   // bathymetry, eta/CC, fixed bed, tides
+    {
+        std::string all(files), one;
+        for (char ch : all + "\n") {
+            if (ch == '\n') {
+                names.push_back(one);
+                one.clear();
+            } else {
+                one += ch;
+            }
+        }
+    }
+    if (state_file.empty() && names.size() == 4 && file_exists(names[0])) {
+        if (names[3] != "0") {
+            std::cerr << "Error: tides input ('" << names[3] << "') is not supported" << std::endl;
+            return EXIT_FAILURE;
+        }
+        std::printf("Reading bathymetry, initial and boundary conditions\n");
+        std::fflush(stdout);
+        unsigned long T = 0;
+        if (tsu_input_time_levels(names[1].c_str(), &T) != TSU_OK) {
+            std::cerr << "Error: " << tsu_last_error() << std::endl;
+            return EXIT_FAILURE;
+        }
+        CC.assign((size_t)(2 * p.npx + 2 * p.npy) * 4 * T, 0.0);
+        tCC.assign(T, 0.0);
+        if (tsu_load_input_files(names[0].c_str(), names[1].c_str(), names[2].c_str(), slr, &p, h.data(), q.data(),
+                                 H.data(), fix.data(), CC.data(), tCC.data(), T) != TSU_OK) {
+            std::cerr << "Error: " << tsu_last_error() << std::endl;
+            return EXIT_FAILURE;
+        }
+    } else if (!state_file.empty()) {
         std::ifstream f(state_file, std::ios::binary);
         std::vector<double> V(5 * nv);
         if (!f.read((char*)V.data(), (std::streamsize)(V.size() * sizeof(double))) ||
@@ -119,8 +153,9 @@ int main(int argc, char** argv)
     }
     std::printf("%i\n", p.nvol);
     double seconds = 0.0;
-    int err = TSUNER(h.data(), q.data(), H.data(), fix.data(), p, prefix, nullptr, nullptr, nullptr, 0, nullptr,
-                     nullptr, &seconds, device, nullptr, nullptr, 0);
+    int err = TSUNER(h.data(), q.data(), H.data(), fix.data(), p, prefix, nullptr, CC.empty() ? nullptr : CC.data(),
+                     tCC.empty() ? nullptr : tCC.data(), (unsigned long)tCC.size(), nullptr, nullptr, &seconds, device,
+                     nullptr, nullptr, 0);
     if (err > 0) {
         if (err == 1)
             std::cerr << "Error: Not enough GPU memory" << std::endl;
