@@ -77,3 +77,24 @@ def coastal_case(tmp, nlon=90, nlat=70, T=8, seed=0, fill=False, upper=True):
     write_series(stem, flon, flat, times, h.astype(np.float32), u.astype(np.float32), v.astype(np.float32),
                  upper=upper)
     return bpath, stem + "h.nc"
+
+
+WESTPORT = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden", "westport_100x100.npz")
+
+
+def westport_scenario():
+    """The reference's example case (data.dat with the westport files, 100 x
+    100 cells) from the committed fixture: Read_files = 1 configuration,
+    inflow series on all sides."""
+    from tsuner_amd import scenarios as S
+    z = np.load(WESTPORT)
+    dd = dict(zip([str(k) for k in z["data_dat_keys"]], z["data_dat"]))
+    npx, npy = int(dd["npx"]), int(dd["npy"])
+    xmin, xmax, ymin, ymax, dx, dy = z["extent"]
+    p = S.base_params(npx, npy, dx, dy, friction=dd["FR"], R=dd["R"], DS=dd["DS"], porosity=dd["porosity"],
+                      OC=dd["OC"], Dsg=dd["Dsg"], CFL=dd["CFL"], epsilon_h=dd["epsilon_h"],
+                      total_time=dd["total_time"], save_time=dd["save_time"])
+    p.update(xmin=xmin, xmax=xmax, ymin=ymin, ymax=ymax, dx=dx, dy=dy)
+    V = S.pack_state(z["h"], 0 * z["h"], 0 * z["h"], z["H"])
+    return S.Scenario(name="westport_100x100", params=p, scheme=S.default_scheme(bc=[S.BC_INFLOW] * 4), V=V,
+                      fixH=z["fixH"], CC=z["CC"], tCC=z["tCC"])

@@ -15,7 +15,7 @@ from tsuner_amd import ingest
 from tsuner_amd import scenarios as S
 from tsuner_amd.solver import Solver
 
-from ingest_files import coastal_case
+from ingest_files import coastal_case, westport_scenario
 
 pytestmark = pytest.mark.gpu
 
@@ -58,6 +58,26 @@ def test_loaded_case_matches_oracle(tmp_path, oracle, npx, npy):
         t, dt, _ = g.time()
         assert t == o.t and dt == o.dt
     assert o.t > tCC[-2]
+
+
+def test_westport_example_matches_oracle(oracle):
+    """The reference's own example (data.dat: westport bathymetry and NCTR
+    series on 100 x 100 cells, committed as the loaded fixture): 400 steps,
+    about 1580 s, inflow until 270 s, wet/dry coast and moving bed."""
+    sc = westport_scenario()
+    say("westport 100x100 from fixture")
+    o = oracle.from_scenario(sc)
+    g = Solver.from_scenario(sc)
+    assert g.dt == o.dt
+    for c in range(4):
+        o.step(100)
+        g.step(100)
+        a, w = o.state(), g.state()
+        assert np.isfinite(a).all()
+        np.testing.assert_array_equal(a, w, err_msg="westport after %d steps" % (100 * (c + 1)))
+        t, dt, _ = g.time()
+        assert t == o.t and dt == o.dt
+    assert o.t > sc.tCC[-2] and np.abs(a[4]).max() > 0   # past the inflow window, sediment in suspension
 
 
 def test_cli_reads_input_files(tmp_path, oracle):

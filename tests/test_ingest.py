@@ -223,3 +223,17 @@ def test_reference_example_files(npx, npy):
     assert abs(lib_out["params"]["xmax"] - OI.dy_transform() * math.cos(math.radians(lat_s)) * span) < 1e-3
     np.testing.assert_array_equal(lib_out["tCC"], 30.0 * np.arange(1, 11))
     assert lib_out["H"].min() > -117 and lib_out["H"].max() < 50
+
+
+@pytest.mark.skipif(not os.path.isdir(EXAMPLE), reason="reference example files not present")
+def test_westport_fixture_matches_the_library():
+    """tests/golden/westport_100x100.npz (written by make_westport.py with the
+    oracle) is what the library loads from the example files today."""
+    from ingest_files import WESTPORT, westport_scenario
+    z = np.load(WESTPORT)
+    lib_out = ingest.load_input_files(os.path.join(EXAMPLE, "westport_bathy_example.nc"),
+                                      os.path.join(EXAMPLE, "new_westport_runupB_h.nc"), params(100, 100))
+    for k in ("h", "H", "fixH", "CC", "tCC"):
+        np.testing.assert_array_equal(lib_out[k], z[k], err_msg=k)
+    sc = westport_scenario()
+    assert sc.params["dx"] == lib_out["params"]["dx"] and sc.params["FR"] == 2.5e-2 / 8
