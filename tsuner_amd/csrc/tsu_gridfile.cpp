@@ -45,27 +45,49 @@ uint64_t be(const uint8_t* p, int n)
     return v;
 }
 
-double decode(const uint8_t* p, int t)
+// n big-endian values of type t from raw to out, converted to double.
+void decode_all(const uint8_t* raw, size_t n, int t, double* out)
 {
     switch (t) {
-    case NC_BYTE: return (double)(int8_t)p[0];
-    case NC_CHAR: return (double)p[0];
-    case NC_SHORT: return (double)(int16_t)be(p, 2);
-    case NC_INT: return (double)(int32_t)be(p, 4);
-    case NC_FLOAT: {
-        const uint32_t u = (uint32_t)be(p, 4);
-        float f;
-        std::memcpy(&f, &u, 4);
-        return (double)f;
+    case NC_BYTE:
+        for (size_t i = 0; i < n; i++) out[i] = (double)(int8_t)raw[i];
+        break;
+    case NC_CHAR:
+        for (size_t i = 0; i < n; i++) out[i] = (double)raw[i];
+        break;
+    case NC_SHORT:
+        for (size_t i = 0; i < n; i++) {
+            uint16_t u;
+            std::memcpy(&u, raw + 2 * i, 2);
+            out[i] = (double)(int16_t)__builtin_bswap16(u);
+        }
+        break;
+    case NC_INT:
+        for (size_t i = 0; i < n; i++) {
+            uint32_t u;
+            std::memcpy(&u, raw + 4 * i, 4);
+            out[i] = (double)(int32_t)__builtin_bswap32(u);
+        }
+        break;
+    case NC_FLOAT:
+        for (size_t i = 0; i < n; i++) {
+            uint32_t u;
+            std::memcpy(&u, raw + 4 * i, 4);
+            u = __builtin_bswap32(u);
+            float f;
+            std::memcpy(&f, &u, 4);
+            out[i] = (double)f;
+        }
+        break;
+    case NC_DOUBLE:
+        for (size_t i = 0; i < n; i++) {
+            uint64_t u;
+            std::memcpy(&u, raw + 8 * i, 8);
+            u = __builtin_bswap64(u);
+            std::memcpy(out + i, &u, 8);
+        }
+        break;
     }
-    case NC_DOUBLE: {
-        const uint64_t u = be(p, 8);
-        double d;
-        std::memcpy(&d, &u, 8);
-        return d;
-    }
-    }
-    return 0.0;
 }
 
 // Forward-only cursor over the file, reading more as the header needs it.
@@ -165,12 +187,8 @@ H5& h5_state()
     return h;
 }
 
-H5* hdf5()
+bool load_hdf5(H5& h)
 {
-    H5& h = h5_state();
-    static bool tried = false;
-    if (tried) return h.so ? &h : nullptr;
-    tried = true;
     std::vector<std::string> names;
     if (const char* e = std::getenv("TSU_HDF5_LIB")) names.push_back(e);
     for (const char* n : {"libhdf5.so", "libhdf5.so.103", "/opt/conda/lib/libhdf5.so"}) names.push_back(n);
@@ -178,7 +196,7 @@ H5* hdf5()
         if ((h.so = dlopen(n.c_str(), RTLD_NOW | RTLD_LOCAL))) break;
     if (!h.so) {
         h.error = "the HDF5 C library (libhdf5.so) is not loadable; set TSU_HDF5_LIB to its path";
-        return nullptr;
+        return false;
     }
     bool ok = sym(h.so, "H5open", &h.open) && sym(h.so, "H5Eset_auto2", &h.eset_auto) &&
               sym(h.so, "H5Fopen", &h.fopen) && sym(h.so, "H5Fclose", &h.fclose) &&
@@ -192,11 +210,19 @@ H5* hdf5()
         h.error = "the loaded HDF5 library lacks the 1.10 C API";
         dlclose(h.so);
         h.so = nullptr;
-        return nullptr;
+        return false;
     }
     h.native_double = *nd;
     h.eset_auto(0, nullptr, nullptr);   // no error stack printing; failures are returned
-    return &h;
+    return true;
+}
+
+// The library, loaded once per process (thread-safe static initialisation);
+// nullptr when unavailable.
+H5* hdf5()
+{
+    static const bool ok = load_hdf5(h5_state());
+    return ok ? &h5_state() : nullptr;
 }
 
 }  // namespace
@@ -431,8 +457,7 @@ bool GridFile::read_double(const std::string& name, std::vector<double>* out) co
             err_ = "short read of " + name;
             return false;
         }
-        double* o = out->data() + r * per;
-        for (size_t i = 0; i < per; i++) o[i] = decode(raw.data() + i * (size_t)ts, v->type);
+        decode_all(raw.data(), per, v->type, out->data() + r * per);
     }
     return true;
 }

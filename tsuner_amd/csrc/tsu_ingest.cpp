@@ -253,6 +253,9 @@ std::string sibling(const std::string& hfile, char c)
 
 int fail(const std::string& msg) { return tsu::set_error(TSU_ERR_HOST, msg); }
 
+// Grid spacings divide source offsets into indices: positive and finite.
+bool spacing_ok(double d) { return std::isfinite(d) && d > 0.0; }
+
 bool coords(const tsu::GridFile& g, const std::vector<std::string>& names, std::vector<double>* v)
 {
     const std::string n = g.find_var(names);
@@ -310,6 +313,8 @@ TSU_API int tsu_load_input_files(const char* bathy_file, const char* eta_file, c
     if (nxb < 2 || nyb < 2 || bat.size() != (size_t)nxb * nyb)
         return tsu::set_error(TSU_ERR_ARG, std::string(bathy_file) + ": bathymetry must be a (lat, lon) grid of at least 2 x 2");
     const double dxb = (lon[nxb - 1] - lon[0]) / (nxb - 1), dyb = (lat[nyb - 1] - lat[0]) / (nyb - 1);
+    if (!spacing_ok(dxb) || !spacing_ok(dyb))
+        return tsu::set_error(TSU_ERR_ARG, std::string(bathy_file) + ": coordinates must increase");
     double xmin = lon[0] - 0.5 * dxb, xmax = lon[nxb - 1] + 0.5 * dxb;
     double ymin = lat[0] - 0.5 * dyb, ymax = lat[nyb - 1] + 0.5 * dyb;
     std::vector<double> H_s(nvol), fix_s, eta_s(nvol);
@@ -324,6 +329,8 @@ TSU_API int tsu_load_input_files(const char* bathy_file, const char* eta_file, c
         const int nxf = (int)flon.size(), nyf = (int)flat.size();
         if (nxf < 2 || nyf < 2 || fx.size() != (size_t)nxf * nyf)
             return tsu::set_error(TSU_ERR_ARG, std::string(fixed_bed_file) + ": map must be a (lat, lon) grid of at least 2 x 2");
+        if (!spacing_ok(flon[nxf - 1] - flon[0]) || !spacing_ok(flat[nyf - 1] - flat[0]))
+            return tsu::set_error(TSU_ERR_ARG, std::string(fixed_bed_file) + ": coordinates must increase");
         fix_s.resize(nvol);
         resample_box(xmin, xmax, ymin, ymax, flon[0], flat[0], nxf, nyf, (flon[nxf - 1] - flon[0]) / (nxf - 1),
                      (flat[nyf - 1] - flat[0]) / (nyf - 1), npx, npy, fx.data(), fix_s.data());
@@ -354,6 +361,8 @@ TSU_API int tsu_load_input_files(const char* bathy_file, const char* eta_file, c
     const size_t lev = F.nx * F.ny;
     if (F.T < 1 || F.nx < 2 || F.ny < 2 || F.h.size() != F.T * lev)
         return tsu::set_error(TSU_ERR_ARG, std::string(eta_file) + ": expected (time, lat, lon) free-surface levels");
+    if (!spacing_ok(F.dx) || !spacing_ok(F.dy))
+        return tsu::set_error(TSU_ERR_ARG, std::string(eta_file) + ": coordinates must increase");
     for (int c = 0; c < 2; c++) {
         const std::string path = sibling(eta_file, c == 0 ? 'u' : 'v');
         tsu::GridFile g;
