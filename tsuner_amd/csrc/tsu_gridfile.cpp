@@ -131,7 +131,7 @@ struct Cursor {
     {
         const uint64_t tag = u(4), count = u(4);
         if (tag == 0 && count == 0) return;
-        if (tag != 0x0C) {
+        if (tag != 0x0C || count > (1u << 20)) {
             ok = false;
             return;
         }
@@ -318,7 +318,7 @@ bool GridFile::open(const std::string& path)
     // dimensions
     uint64_t tag = c.u(4), count = c.u(4);
     if (!(tag == 0 && count == 0)) {
-        if (tag != 0x0A) c.ok = false;
+        if (tag != 0x0A || count > (1u << 20)) c.ok = false;
         for (uint64_t i = 0; i < count && c.ok; i++) {
             Impl::Dim d;
             d.name = c.name();
@@ -330,7 +330,7 @@ bool GridFile::open(const std::string& path)
     tag = c.u(4);
     count = c.u(4);
     if (!(tag == 0 && count == 0)) {
-        if (tag != 0x0B) c.ok = false;
+        if (tag != 0x0B || count > (1u << 20)) c.ok = false;
         for (uint64_t i = 0; i < count && c.ok; i++) {
             Impl::Var v;
             v.name = c.name();
