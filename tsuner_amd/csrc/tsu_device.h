@@ -381,29 +381,29 @@ __device__ __forceinline__ void riemann(const DevParams& P, bool bed_mobile,
         SR = u1.x + sg * sh1;
         cm = sg * rsqrt_(hm);
     }
+    // h0 + h1 > 0 here, so sqrt(h0) + sqrt(h1) > 0: the Roe velocity is
+    // always defined (the reference's 0/0 of two dry sides exits above).
     R ssum = sh0 + sh1;
-    if (ssum > zero) {  // both sides dry: the reference's 0/0 is dropped by fmin/fmax
-        R umx = rdiv(sh0 * u0.x + sh1 * u1.x, ssum);
-        SL = rmin(SL, umx - cm);
-        SR = rmax(SR, umx + cm);
-    }
+    R umx = rdiv(sh0 * u0.x + sh1 * u1.x, ssum);
+    SL = rmin(SL, umx - cm);
+    SR = rmax(SR, umx + cm);
 
-    R a0, a1;
+    // PVM coefficients; both branches of VolumeSolver.cu:75-83 evaluated and
+    // selected (no block split), the division on a denominator kept in range.
     R dS = SR - SL;
-    if (dS > (R)kEpsilon) {
-        if constexpr (sizeof(R) == 8) {
-            const Den<double> D = den(dS);
-            a0 = qdiv(SR * fabs(SL) - SL * fabs(SR), D);
-            a1 = qdiv(fabs(SR) - fabs(SL), D);
-        } else {
-            R r = __builtin_amdgcn_rcpf(dS);
-            a0 = (SR * rabs(SL) - SL * rabs(SR)) * r;
-            a1 = (rabs(SR) - rabs(SL)) * r;
-        }
+    const bool spread = dS > (R)kEpsilon;
+    R a0, a1;
+    if constexpr (sizeof(R) == 8) {
+        const Den<double> D = den(spread ? dS : 1.0);
+        a0 = qdiv(SR * fabs(SL) - SL * fabs(SR), D);
+        a1 = qdiv(fabs(SR) - fabs(SL), D);
     } else {
-        a0 = rmax(rabs(SL), rabs(SR));
-        a1 = zero;
+        R r = __builtin_amdgcn_rcpf(spread ? dS : (R)1);
+        a0 = (SR * rabs(SL) - SL * rabs(SR)) * r;
+        a1 = (rabs(SR) - rabs(SL)) * r;
     }
+    a0 = spread ? a0 : rmax(rabs(SL), rabs(SR));
+    a1 = spread ? a1 : zero;
 
     R df0 = F10 - F00, df1 = F11 - F01, df2 = F12 - F02;
     df1 = rmad(g * hm, deta, df1);
