@@ -35,7 +35,7 @@ HBM_PEAK_GBS = 8000.0
 # FETCH_SIZE and WRITE_SIZE runs scaled by what the same counters read for a
 # plain copy kernel of known size (x2.0 and x1.0); the summary with the
 # commands is profiles/stage_kernel_8192_counters.md.  None at other sizes.
-TRAFFIC_8192 = {"fp64": 7.522e9, "fp32": 6.244e9}
+TRAFFIC_8192 = {"fp64": 7.551e9, "fp32": 6.247e9}
 METRIC = "cell-updates/s at 8192² FP64 vs FP32; ensemble sims/hr coarse+NN vs fine"
 
 
