@@ -98,3 +98,45 @@ def westport_scenario():
     V = S.pack_state(z["h"], 0 * z["h"], 0 * z["h"], z["H"])
     return S.Scenario(name="westport_100x100", params=p, scheme=S.default_scheme(bc=[S.BC_INFLOW] * 4), V=V,
                       fixH=z["fixH"], CC=z["CC"], tCC=z["tCC"])
+
+
+def write_series_hdf5(stem, lon, lat, times, h, u, v):
+    """stem + 'h.nc', 'u.nc', 'v.nc' as HDF5 files laid out like the NCTR
+    NetCDF-4 series (root datasets LON, LAT, TIME and HA / UA / VA of shape
+    (T, nlat, nlon), 32-bit floats), written through the HDF5 C library."""
+    import ctypes as C
+    L = None
+    for name in (os.environ.get("TSU_HDF5_LIB"), "libhdf5.so", "/opt/conda/lib/libhdf5.so"):
+        if name:
+            try:
+                L = C.CDLL(name)
+                break
+            except OSError:
+                pass
+    if L is None:
+        raise RuntimeError("libhdf5 not loadable")
+    i64 = C.c_int64
+    L.H5open()
+    L.H5Fcreate.restype, L.H5Fcreate.argtypes = i64, [C.c_char_p, C.c_uint, i64, i64]
+    L.H5Screate_simple.restype, L.H5Screate_simple.argtypes = i64, [C.c_int, C.POINTER(C.c_ulonglong), C.c_void_p]
+    L.H5Dcreate2.restype, L.H5Dcreate2.argtypes = i64, [i64, C.c_char_p, i64, i64, i64, i64, i64]
+    L.H5Dwrite.argtypes = [i64, i64, i64, i64, i64, C.c_void_p]
+    for f in ("H5Dclose", "H5Sclose", "H5Fclose"):
+        getattr(L, f).argtypes = [i64]
+    f32 = C.c_int64.in_dll(L, "H5T_IEEE_F32LE_g").value
+    nat = C.c_int64.in_dll(L, "H5T_NATIVE_FLOAT_g").value
+    paths = []
+    for c, data, var in (("h", h, "HA"), ("u", u, "UA"), ("v", v, "VA")):
+        path = str(stem) + c + ".nc"
+        fid = L.H5Fcreate(path.encode(), 2, 0, 0)   # H5F_ACC_TRUNC
+        for name, arr in (("LON", lon), ("LAT", lat), ("TIME", times), (var, data)):
+            a = np.ascontiguousarray(arr, np.float32)
+            dims = (C.c_ulonglong * a.ndim)(*a.shape)
+            sp = L.H5Screate_simple(a.ndim, dims, None)
+            ds = L.H5Dcreate2(fid, name.encode(), f32, sp, 0, 0, 0)
+            assert ds >= 0 and L.H5Dwrite(ds, nat, 0, 0, 0, a.ctypes.data) >= 0
+            L.H5Dclose(ds)
+            L.H5Sclose(sp)
+        L.H5Fclose(fid)
+        paths.append(path)
+    return paths

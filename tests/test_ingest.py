@@ -13,7 +13,7 @@ import pyingest as OI
 from tsuner_amd import _lib, ingest
 from tsuner_amd import scenarios as S
 
-from ingest_files import coastal_case, write_bathy, write_series
+from ingest_files import coastal_case, write_bathy, write_series, write_series_hdf5
 
 EXAMPLE = "/root/reference/example"
 
@@ -161,6 +161,37 @@ def test_library_matches_oracle_on_synthetic_files(tmp_path, npx, npy, upper, fi
     orc = OI.load_problem_files(b, e, "0", 0.3, npx, npy)
     compare(lib_out, orc)
     assert np.all(lib_out["h"] >= 0) and np.all(np.isfinite(lib_out["CC"]))
+
+
+def _hdf5_available():
+    try:
+        OI._H5.lib()
+        return True
+    except Exception:
+        return False
+
+
+@pytest.mark.skipif(not _hdf5_available(), reason="libhdf5 not loadable")
+def test_library_matches_oracle_on_hdf5_series(tmp_path):
+    """The NetCDF-4 (HDF5) path of the reader, on files written here: the
+    same series as NetCDF classic and as HDF5 load to identical arrays."""
+    b, e = coastal_case(tmp_path, seed=11, T=5)
+    from scipy.io import netcdf_file
+    with netcdf_file(e, "r", mmap=False) as f:
+        lon, lat, t = (np.array(f.variables[k].data) for k in ("LON", "LAT", "TIME"))
+        h = np.array(f.variables["HA"].data)
+    with netcdf_file(e.replace("h.nc", "u.nc"), "r", mmap=False) as f:
+        u = np.array(f.variables["UA"].data)
+    with netcdf_file(e.replace("h.nc", "v.nc"), "r", mmap=False) as f:
+        v = np.array(f.variables["VA"].data)
+    paths = write_series_hdf5(str(tmp_path / "h5_"), lon, lat, t, h, u, v)
+    with open(paths[0], "rb") as f:
+        assert f.read(4) == b"\x89HDF"
+    classic = ingest.load_input_files(b, e, params(70, 55), sea_level_correction=0.05)
+    h5 = ingest.load_input_files(b, paths[0], params(70, 55), sea_level_correction=0.05)
+    compare(h5, OI.load_problem_files(b, paths[0], "0", 0.05, 70, 55))
+    for k in ("h", "H", "CC", "tCC"):
+        np.testing.assert_array_equal(h5[k], classic[k], err_msg=k)
 
 
 def test_fixed_bed_map_as_short_integers(tmp_path):
