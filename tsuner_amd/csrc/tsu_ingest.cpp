@@ -369,13 +369,17 @@ TSU_API int tsu_load_input_files(const char* bathy_file, const char* eta_file, c
         if (!g.open(path)) return fail(g.error());
         const std::string vn = g.find_var(c == 0 ? std::vector<std::string>{"ua_lc", "UA", "ua"}
                                                  : std::vector<std::string>{"va_lc", "VA", "va"});
+        // the first T levels, T from the h file (interpolations.cxx:405-406,447)
         std::vector<double>& dst = c == 0 ? F.u : F.v;
-        if (vn.empty() || !g.read_double(vn, &dst) || dst.size() != F.T * lev)
-            return fail(path + ": no velocity series of the free surface's shape");
+        if (vn.empty() || !g.read_double(vn, &dst) || dst.size() % lev != 0 || dst.size() < F.T * lev)
+            return fail(path + ": no velocity series on the free surface's grid with " + std::to_string(F.T) +
+                        " levels");
+        dst.resize(F.T * lev);
         if (c == 0) {   // the times come from the u file (interpolations.cxx:418-425)
             const std::string tv = g.find_var({"time", "TIME"});
-            if (tv.empty() || !g.read_double(tv, &F.time) || F.time.size() != F.T)
+            if (tv.empty() || !g.read_double(tv, &F.time) || F.time.size() < F.T)
                 return fail(path + ": no time/TIME variable of " + std::to_string(F.T) + " levels");
+            F.time.resize(F.T);
         }
     }
     resample_box(xmin, xmax, ymin, ymax, F.x0, F.y0, (int)F.nx, (int)F.ny, F.dx, F.dy, npx, npy, F.h.data(),
