@@ -128,12 +128,12 @@ __device__ __forceinline__ R lim_vanleer(R a, R b)
 {
     R aa = rabs(a), bb = rabs(b);
     R s = aa + bb;
-    R d = (R)0;
-    if ((a > (R)0 && b > (R)0) || (a < (R)0 && b < (R)0)) {
-        const R p = a * b;                  // > 0 for same-sign slopes
-        if (s > (R)kEpsilon) d = rdiv(a > (R)0 ? p + p : -(p + p), s);
-    }
-    return d;
+    const bool use = ((a > (R)0 && b > (R)0) || (a < (R)0 && b < (R)0)) && s > (R)kEpsilon;
+    const R p = a * b;                      // > 0 for same-sign slopes
+    // evaluated for every lane and selected, so the limiters of a face set
+    // share one basic block; the divisor is 1 where the result is unused
+    const R d = rdiv(a > (R)0 ? p + p : -(p + p), use ? s : (R)1);
+    return use ? d : (R)0;
 }
 
 template <typename R>
