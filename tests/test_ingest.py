@@ -194,6 +194,32 @@ def test_library_matches_oracle_on_hdf5_series(tmp_path):
         np.testing.assert_array_equal(h5[k], classic[k], err_msg=k)
 
 
+def test_longer_velocity_series_use_their_first_levels(tmp_path):
+    """The level count comes from the h file; u, v and the times (read from
+    the u file) contribute their first T levels (interpolations.cxx:405-447)."""
+    b, e = coastal_case(tmp_path, seed=5, T=4)
+    from scipy.io import netcdf_file
+    ref = ingest.load_input_files(b, e, params(40, 30))
+    for c, var in (("u", "UA"), ("v", "VA")):
+        path = e.replace("h.nc", c + ".nc")
+        with netcdf_file(path, "r", mmap=False) as f:
+            lon, lat, t = (np.array(f.variables[k].data) for k in ("LON", "LAT", "TIME"))
+            data = np.array(f.variables[var].data)
+        extra = np.concatenate([data, 100.0 + data[:2]])
+        with netcdf_file(path, "w") as f:
+            f.createDimension("TIME", None)
+            f.createDimension("LAT", len(lat))
+            f.createDimension("LON", len(lon))
+            for k, arr, dims in (("LON", lon, ("LON",)), ("LAT", lat, ("LAT",))):
+                f.createVariable(k, "f4", dims)[:] = arr
+            f.createVariable(var, "f4", ("TIME", "LAT", "LON"))[:] = extra.astype(np.float32)
+            f.createVariable("TIME", "f4", ("TIME",))[:] = np.concatenate([t, [999.0, 1999.0]])
+    longer = ingest.load_input_files(b, e, params(40, 30))
+    compare(longer, OI.load_problem_files(b, e, "0", 0.0, 40, 30))
+    for k in ("h", "H", "CC", "tCC"):
+        np.testing.assert_array_equal(longer[k], ref[k], err_msg=k)
+
+
 def test_fixed_bed_map_as_short_integers(tmp_path):
     b, e = coastal_case(tmp_path, seed=3)
     lon = 235.78 + 0.004 * np.arange(40)
