@@ -50,6 +50,26 @@ def log(msg):
     print(msg, file=sys.stderr, flush=True)
 
 
+class _StdoutToStderr:
+    """Sends file descriptor 1 to stderr for the duration (the library's
+    TSUNER() prints the reference's progress lines on stdout, and this
+    program's stdout carries exactly one JSON line)."""
+
+    def __enter__(self):
+        import ctypes
+        self._libc = ctypes.CDLL(None)
+        sys.stdout.flush()
+        self._saved = os.dup(1)
+        os.dup2(2, 1)
+        return self
+
+    def __exit__(self, *exc):
+        self._libc.fflush(None)
+        os.dup2(self._saved, 1)
+        os.close(self._saved)
+        return False
+
+
 def cpu_model():
     try:
         for line in open("/proc/cpuinfo"):
@@ -191,6 +211,45 @@ def run_ensemble(members, steps, seed=2024, chunk=50):
 BF16_MFMA_PEAK_TFLOPS = 2500.0
 
 
+def run_reference_example():
+    """The reference's own example (its data.dat: westport bathymetry and
+    NCTR series, 100 x 100 cells, 7200 s, output every 20 s), from the loaded
+    inputs committed as tests/golden/westport_100x100.npz: the GPU-resident
+    run to 7200 s (cell-updates/s), then the whole TSUNER() call with its
+    NetCDF output (wall seconds as TSUNER reports them)."""
+    import tempfile
+    from tsuner_amd import scenarios as S
+    from tsuner_amd.solver import Solver, run_tsuner
+    z = np.load(os.path.join(ROOT, "tests", "golden", "westport_100x100.npz"))
+    dd = dict(zip([str(k) for k in z["data_dat_keys"]], z["data_dat"]))
+    npx, npy = int(dd["npx"]), int(dd["npy"])
+    xmin, xmax, ymin, ymax, dx, dy = (float(v) for v in z["extent"])
+    p = S.base_params(npx, npy, dx, dy, friction=dd["FR"], R=dd["R"], DS=dd["DS"], porosity=dd["porosity"],
+                      OC=dd["OC"], Dsg=dd["Dsg"], CFL=dd["CFL"], epsilon_h=dd["epsilon_h"],
+                      total_time=dd["total_time"], save_time=dd["save_time"])
+    p.update(xmin=xmin, xmax=xmax, ymin=ymin, ymax=ymax, dx=dx, dy=dy)
+    V = S.pack_state(z["h"], 0 * z["h"], 0 * z["h"], z["H"])
+    scheme = S.default_scheme(bc=[S.BC_INFLOW] * 4)
+    total = float(dd["total_time"])
+    log("[bench] reference example: %dx%d, %.0f s" % (npx, npy, total))
+    g = Solver(p, scheme, V, z["fixH"], z["CC"], z["tCC"])
+    g.set_stop_time(total)
+    g.synchronize()
+    t0 = time.perf_counter()
+    while g.t < total:
+        g.step(500)
+    wall = time.perf_counter() - t0
+    steps = g.steps
+    g.close()
+    with tempfile.TemporaryDirectory() as d, _StdoutToStderr():
+        code, secs = run_tsuner(p, V, z["fixH"], os.path.join(d, "sim"), z["CC"], z["tCC"])
+    return {"workload": "data.dat example: westport, %dx%d cells (dx %.1f m, dy %.1f m), %.0f s, inflow series "
+                        "on all sides, sediment on, FP64" % (npx, npy, dx, dy, total),
+            "steps": steps, "gpu_resident_s": wall, "cell_updates_per_s": npx * npy * steps / wall,
+            "tsuner_call_s": secs if code == 0 else None,
+            "tsuner_call": "TSUNER() with NetCDF output every %.0f s" % dd["save_time"]}
+
+
 def run_correction(heldout, fine_steps):
     """Coarse + learned correction against the fine solver on held-out
     Gaussian-source scenarios (1024^2 fine, 256^2 coarse), with the shipped
@@ -264,6 +323,7 @@ def main():
     ap.add_argument("--ensemble-members", type=int, default=256)
     ap.add_argument("--ensemble-steps", type=int, default=3000)
     ap.add_argument("--no-correction", action="store_true", help="skip the coarse + learned-correction leg")
+    ap.add_argument("--no-example", action="store_true", help="skip the reference's data.dat example case")
     ap.add_argument("--correction-scenarios", type=int, default=64)
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
@@ -295,6 +355,12 @@ def main():
                   "state": "initial state plus u = 1 m/s, v = 0.25 m/s and c = 1e-3 in every wet cell"}
         del sa
     small = None if args.no_small else run_small_grids(args.warmup)
+    example = None
+    if not args.no_example:
+        try:
+            example = run_reference_example()
+        except Exception as e:   # informative leg: reported, never fatal to the headline line
+            log("[bench] reference example failed: %r" % (e,))
     config_laws = None
     if not args.no_small:
         # the configurations' own laws where they differ from the reference's:
@@ -355,6 +421,7 @@ def main():
         "pcie_inclusive": r64["pcie_inclusive"],
         "active_flow": active,
         "small_grids": small,
+        "reference_example": example,
         "config_laws": config_laws,
         "ensemble_fine": ensemble,
         "ensemble_coarse_nn": correction,
