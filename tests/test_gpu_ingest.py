@@ -138,3 +138,19 @@ def test_cli_reads_input_files(tmp_path, oracle):
             np.testing.assert_array_equal(f.variables["eta"][k], W[0] - W[3])
             np.testing.assert_array_equal(f.variables["ux"][k], W[1] / (W[0] + EPS))
             np.testing.assert_array_equal(f.variables["b"][k], -W[3])
+
+
+def test_cli_rejects_tides_input(tmp_path):
+    """A tides file in data.dat (the reference's unfinished tides input) is
+    refused with a message and a failing exit status, before any run."""
+    b, e = coastal_case(tmp_path, T=4, seed=6)
+    lines = ["# data.dat with a tides file", "# xmin", "0", "# xmax", "1", "# ymin", "0", "# ymax", "1",
+             "# total time", "60", "# save time", "20", "# npx", "32", "# npy", "24",
+             "# CFL", "0.9", "# epsilon_h", "1e-4", "# R", "2.65", "# DS", "2e-4", "# friction", "2.5e-2",
+             "# porosity", "0.4", "# critical theta", "0.0047", "# Dsg", "100e-6", "# output", "tide_run",
+             "# bathymetry", os.path.basename(b), "# eta", os.path.basename(e), "# fixed bed", "0",
+             "# sea level", "0", "# tides", "tides.nc"]
+    (tmp_path / "data.dat").write_text("\n".join(lines) + "\n")
+    r = subprocess.run([CLI, "data.dat", "0"], cwd=tmp_path, capture_output=True, text=True, timeout=60)
+    assert r.returncode != 0 and "tides" in r.stderr
+    assert not (tmp_path / "tide_run.nc").exists()
