@@ -256,6 +256,52 @@ __device__ __forceinline__ Vel<float> velocity(float h, float qx, float qy)
 // Velocity and concentration hc / (h + DBL_EPSILON) of one state
 // (SedFlux.cuh:26-49, 103-132); in FP64 the three quotients share one
 // reciprocal of h + DBL_EPSILON.
+// The speed cap of velocity64 on an uncapped (u.x, u.y): |u| > 10 m/s is
+// scaled to 10 m/s, and u.s set.
+__device__ __forceinline__ void cap_speed(Vel<double>& u)
+{
+    double s = u.x * u.x + u.y * u.y;
+    if (s > kVmax2Cut) {
+        const Den<double> Dm = den(sqrt(s));
+        u.x = qdiv(u.x, Dm) * kVmax;
+        u.y = qdiv(u.y, Dm) * kVmax;
+        s = u.x * u.x + u.y * u.y;
+    }
+    u.s = s;
+}
+
+// velocity_conc for the two sides of an edge at once (values identical):
+// one wave-wide test whether any lane of either side divides, then both
+// sides' reciprocals and quotients for every lane in one basic block, the
+// unused ones discarded by selects (h + DBL_EPSILON is a valid shared
+// denominator for any face depth).
+template <bool SED>
+__device__ __forceinline__ void velocity_conc_pair(double h0, double qx0, double qy0, double hc0, double h1,
+                                                   double qx1, double qy1, double hc1, Vel<double>& u0, double& c0,
+                                                   Vel<double>& u1, double& c1)
+{
+    const bool nu0 = h0 >= kEps && (qx0 != 0.0 || qy0 != 0.0), nc0 = SED && hc0 != 0.0;
+    const bool nu1 = h1 >= kEps && (qx1 != 0.0 || qy1 != 0.0), nc1 = SED && hc1 != 0.0;
+    u0.x = u0.y = u0.s = 0.0;
+    u1.x = u1.y = u1.s = 0.0;
+    c0 = c1 = 0.0;
+    if (__builtin_amdgcn_ballot_w64(nu0 || nc0 || nu1 || nc1) == 0) return;
+    const Den<double> D0 = den(h0 + kEpsilon), D1 = den(h1 + kEpsilon);
+    const double x0 = qdiv(qx0, D0), y0 = qdiv(qy0, D0);
+    const double x1 = qdiv(qx1, D1), y1 = qdiv(qy1, D1);
+    if (SED) {
+        const double k0 = qdiv(hc0, D0), k1 = qdiv(hc1, D1);
+        c0 = nc0 ? k0 : 0.0;
+        c1 = nc1 ? k1 : 0.0;
+    }
+    u0.x = (nu0 && qx0 != 0.0) ? x0 : 0.0;
+    u0.y = (nu0 && qy0 != 0.0) ? y0 : 0.0;
+    u1.x = (nu1 && qx1 != 0.0) ? x1 : 0.0;
+    u1.y = (nu1 && qy1 != 0.0) ? y1 : 0.0;
+    cap_speed(u0);
+    cap_speed(u1);
+}
+
 template <bool SED>
 __device__ __forceinline__ void velocity_conc(double h, double qx, double qy, double hc, Vel<double>& u, double& c)
 {
@@ -357,8 +403,12 @@ __device__ __forceinline__ void riemann(const DevParams& P, bool bed_mobile,
     }
     Vel<R> u0, u1;
     R c0, c1;
-    velocity_conc<SED>(hl, qnl, qtl, hcl, u0, c0);
-    velocity_conc<SED>(hr, qnr, qtr, hcr, u1, c1);
+    if constexpr (sizeof(R) == 8) {
+        velocity_conc_pair<SED>(hl, qnl, qtl, hcl, hr, qnr, qtr, hcr, u0, c0, u1, c1);
+    } else {
+        velocity_conc<SED>(hl, qnl, qtl, hcl, u0, c0);
+        velocity_conc<SED>(hr, qnr, qtr, hcr, u1, c1);
+    }
     R q0n = u0.x * h0, q0t = u0.y * h0;
     R q1n = u1.x * h1, q1t = u1.y * h1;
     R deta = h1 - h0;
