@@ -35,13 +35,13 @@ HBM_PEAK_GBS = 8000.0
 # FETCH_SIZE and WRITE_SIZE runs scaled by what the same counters read for a
 # plain copy kernel of known size (x2.0 and x1.0); the summary with the
 # commands is profiles/stage_kernel_8192_counters.md.  None at other sizes.
-TRAFFIC_8192 = {"fp64": 7.558e9, "fp32": 6.266e9}
+TRAFFIC_8192 = {"fp64": 7.595e9, "fp32": 6.262e9}
 # The stage kernels are bound by VALU issue, not HBM.  VALU instructions per
 # wave-row (one 64-lane wave over 60 cells of one row, one stage) on the
 # 8192^2 sediment workload, from rocprofv3 SQ_INSTS_VALU / (cells / 60) in the
 # same profile; the issue peak is MI355X's 1024 SIMDs x 2.4 GHz / 4 cycles per
 # wave64 VALU instruction (its published 78.6 TFLOP/s FP64 vector rate).
-VALU_PER_WAVE_ROW_8192 = {"fp64": 1383.0, "fp32": 1073.0}
+VALU_PER_WAVE_ROW_8192 = {"fp64": 1357.0, "fp32": 1073.0}
 VALU_ISSUE_PEAK = 1024 * 2.4e9 / 4
 METRIC = "cell-updates/s at 8192² FP64 vs FP32; ensemble sims/hr coarse+NN vs fine"
 
