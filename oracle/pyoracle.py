@@ -12,7 +12,10 @@ import subprocess
 import numpy as np
 
 HERE = os.path.dirname(os.path.abspath(__file__))
-LIB_PATH = os.path.join(HERE, "libtsu_oracle.so")
+# TSU_ORACLE_LIB names another build of the same sources (the mutation
+# self-test of tests/test_oracle_mutations.py loads deliberately broken
+# copies through it, to show that the pins can tell).
+LIB_PATH = os.environ.get("TSU_ORACLE_LIB") or os.path.join(HERE, "libtsu_oracle.so")
 
 _dp = C.POINTER(C.c_double)
 _ip = C.POINTER(C.c_int)
@@ -96,6 +99,8 @@ def lib():
         L.orc_erosion_deposition.argtypes = [C.POINTER(OrcSimParams), C.POINTER(OrcScheme), C.c_double, C.c_double,
                                              C.c_double, C.c_double, _dp]
         L.orc_derive_params.argtypes = [C.POINTER(OrcSimParams)]
+        L.orc_cell_faces.restype = C.c_double
+        L.orc_cell_faces.argtypes = [C.c_int, _dp, _dp, _dp]
         L.orc_shared_quotient_mismatches.restype = C.c_long
         L.orc_shared_quotient_mismatches.argtypes = [C.c_long, C.c_ulonglong]
         _lib = L
@@ -220,6 +225,15 @@ def cell_dt(params: dict, h, qx, qy) -> float:
 def min_reduction(dtv: np.ndarray, narrow: int = 1) -> float:
     a = np.ascontiguousarray(dtv, dtype=np.float64)
     return lib().orc_min_reduction(_ptr(a), a.size, narrow)
+
+
+def cell_faces(limiter: int, minus, cell, plus):
+    """Filtered faces (Fm, Fp) of the middle cell along one axis and its
+    limited surface slope deta; each argument is (h, q_axis, q_other, H, hc)."""
+    c3 = np.ascontiguousarray(np.concatenate([minus, cell, plus]), dtype=np.float64)
+    fm, fp = np.zeros(5), np.zeros(5)
+    deta = lib().orc_cell_faces(int(limiter), _ptr(c3), _ptr(fm), _ptr(fp))
+    return fm, fp, deta
 
 
 def erosion_rate(params: dict, scheme: dict, h, qx, qy, hc) -> float:

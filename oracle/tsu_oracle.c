@@ -451,6 +451,29 @@ static double o_recons_dir(orc_sim *S, long pos, long pl, long pr, double uix, d
     return deta;
 }
 
+/* One cell's two faces along one axis, as the reconstructor forms them
+ * (Reconstructor.cu:120-197 for x, :203-274 for y, then the face filter of
+ * :345-348).  c3 holds the minus-side neighbour, the cell and the plus-side
+ * neighbour (h, q_axis, q_other, H, hc each, in the axis frame); the cell's
+ * velocity is formed as Reconstructor.cu:112-118 does.  Returns deta. */
+double orc_cell_faces(int limiter, const double c3[15], double Fm[5], double Fp[5])
+{
+    orc_sim S;
+    double V[15], ux, uy, deta;
+    int f, k;
+    memset(&S, 0, sizeof(S));
+    for (f = 0; f < 5; f++)
+        for (k = 0; k < 3; k++) V[f * 3 + k] = c3[k * 5 + f];
+    S.nvol = 3;
+    S.V = V;
+    S.s.limiter = limiter;
+    orc_velocity(V[1], V[3 + 1], V[6 + 1], &ux, &uy);
+    deta = o_recons_dir(&S, 1, 0, 2, ux, uy, Fm, Fp);
+    orc_state_filter(&Fm[0], &Fm[1], &Fm[2], &Fm[4]);
+    orc_state_filter(&Fp[0], &Fp[1], &Fp[2], &Fp[4]);
+    return deta;
+}
+
 static void o_put5(double *A, long nv, long pos, const double *v)
 {
     int f;

@@ -104,6 +104,9 @@ double orc_cell_dt(const orc_sim_params *p, double h, double qx, double qy);
 double orc_min_reduction(const double *dtv, long n, int narrow);
 void   orc_erosion_deposition(const orc_sim_params *p, const orc_scheme *s,
                               double h, double qx, double qy, double hc, double *phi);
+/* One cell's filtered faces along one axis from (minus neighbour, cell,
+ * plus neighbour), 5 values each in the axis frame; returns deta. */
+double orc_cell_faces(int limiter, const double c3[15], double Fm[5], double Fp[5]);
 /* Derive Q, Opar, HQB and FR/8 from data.dat values (Problem_loader.cxx:337-342). */
 void   orc_derive_params(orc_sim_params *p);
 /* Test support (tsu_quotient.c): seeded check of the shared-denominator

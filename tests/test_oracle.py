@@ -97,8 +97,10 @@ def test_bedload_ashida_michiue_textbook(oracle, sed):
 def test_bedload_grass(oracle, sed):
     p, s = sed
     s = dict(s, bedload_law=S.BEDLOAD_GRASS, grass_ag=1e-3)
-    got = oracle.bedload_flux(p, s, 1.0, 0.6, 0.8)
-    assert got == pytest.approx(-1e-3 / (1 - p["porosity"]) * 0.6 * 1.0, rel=1e-14)
+    # q_b = A_g / (1 - p) u_n |u|^2 (|u| = 1.5 here, so |u| and |u|^2 differ)
+    got = oracle.bedload_flux(p, s, 1.0, 1.2, 0.9)
+    assert got == pytest.approx(-1e-3 / (1 - p["porosity"]) * 1.2 * 1.5 ** 2, rel=1e-14)
+    assert oracle.bedload_flux(p, s, 5 * p["DS"] * 0.99, 1.2, 0.9) == 0.0
 
 
 def test_erosion_deposition_garcia_parker(oracle, sed):
@@ -270,6 +272,26 @@ def test_uniform_periodic_state_friction_and_exchange_closed_form(oracle):
     assert np.allclose(W[3] + W[4], H + hc, rtol=1e-15)
 
 
+def test_uniform_periodic_state_manning_closed_form(oracle):
+    """The Manning law of the plain-SWE configuration, semi-implicit like the
+    reference's quadratic law: q <- q / (1 + c) with
+    c = dt g n^2 |u| / h^(4/3), from U^n in both stages."""
+    n = 6
+    p = S.base_params(n, n, 10.0, 10.0)
+    s = S.default_scheme(bc=[S.BC_PERIODIC] * 4, bedload=0, suspension=0, friction_law=S.FRIC_MANNING,
+                         manning_n=0.03)
+    h, q = 0.7, 1.1
+    V = S.pack_state(np.full((n, n), h), np.full((n, n), q), np.zeros((n, n)), np.full((n, n), 2.0))
+    o = oracle.OracleSim(p, s, V, np.ones((n, n), np.int32))
+    dt = o.dt
+    eps = np.finfo(float).eps
+    u = q / (h + eps)
+    c = dt * G * 0.03 ** 2 * u / (h + eps) ** (4.0 / 3.0)
+    o.step(1)
+    qn = 0.5 * q + 0.5 * q / (1 + c) / (1 + c)
+    assert np.allclose(o.state()[1], qn * h / (h + eps), rtol=1e-14, atol=0)
+
+
 def test_lake_at_rest_grid(oracle):
     sc = S.lake_at_rest(n=48)
     o = oracle.from_scenario(sc)
@@ -438,3 +460,378 @@ def test_shared_denominator_quotient_is_the_ieee_quotient(oracle):
     d = np.abs(rng.normal(size=100000)) + 1e-3
     x = rng.normal(size=100000)
     assert np.any(x * (1.0 / d) != x / d)
+
+
+# ------------------------------------------- face reconstruction, one cell --
+
+def _triple(h, u, v, H, c):
+    """(minus, cell, plus) states from primitive values at k = -1, 0, 1."""
+    return [np.array([h[k], h[k] * u[k], h[k] * v[k], H[k], h[k] * c[k]]) for k in range(3)]
+
+
+@pytest.mark.parametrize("lim", [S.LIM_VANLEER, S.LIM_MINMOD])
+def test_faces_reproduce_linear_data(oracle, lim):
+    """Second-order MUSCL on data whose primitive variables (h, u, v, H, c)
+    vary linearly: every limiter with phi(s, s) = s returns the slope itself,
+    so depth, bed and concentration faces are the values at the faces, the
+    face hc is c(face) * h(face), and the face momentum is the first-order
+    Taylor expansion of h*u about the centre (product rule: h u' + u h')."""
+    rng = np.random.default_rng(11)
+    for _ in range(400):
+        h0, a = rng.uniform(1, 5), rng.uniform(-0.4, 0.4)
+        u0, b = rng.uniform(-2, 2), rng.uniform(-0.3, 0.3)
+        v0, e = rng.uniform(-2, 2), rng.uniform(-0.3, 0.3)
+        H0, dH = rng.uniform(-2, 8), rng.uniform(-0.5, 0.5)
+        c0, dc = rng.uniform(0.01, 0.05), rng.uniform(-0.004, 0.004)
+        k = np.array([-1.0, 0.0, 1.0])
+        m, c, p = _triple(h0 + a * k, u0 + b * k, v0 + e * k, H0 + dH * k, c0 + dc * k)
+        # the surface must stay wet across the triple for the hydrostatic
+        # depths to be the depths themselves
+        fm, fp, deta = oracle.cell_faces(lim, m, c, p)
+        for s, f in ((-0.5, fm), (0.5, fp)):
+            hf, cf = h0 + s * a, c0 + s * dc
+            assert f[0] == pytest.approx(hf, rel=1e-13)
+            assert f[3] == pytest.approx(H0 + s * dH, rel=1e-13, abs=1e-13)
+            assert f[4] == pytest.approx(cf * hf, rel=1e-11)
+            assert f[1] == pytest.approx(h0 * u0 + s * (h0 * b + u0 * a), rel=1e-12, abs=1e-12)
+            assert f[2] == pytest.approx(h0 * v0 + s * (h0 * e + v0 * a), rel=1e-12, abs=1e-12)
+        # the limited surface slope is the slope of eta = h - H
+        assert deta == pytest.approx(a - dH, rel=1e-12, abs=1e-13)
+
+
+def test_faces_depth_positivity_and_mean(oracle):
+    """Whatever the neighbours, the two depth faces of a cell lie in
+    [0, 2 h_i] and average to h_i (Reconstructor.cu:150-152 bounds the depth
+    slope so that the reconstruction is positivity-preserving and
+    conservative); concentration faces are non-negative.  Neighbour depths
+    are drawn wet, dry and (as a careless caller could pass) negative,
+    which is what drives the limited slope past 2 h_i."""
+    rng = np.random.default_rng(12)
+    clipped = 0
+    for it in range(3000):
+        hi = rng.choice([rng.uniform(0, 1e-3), rng.uniform(0, 3)])
+        hs = [rng.choice([0.0, rng.uniform(0, 6), -rng.uniform(0, 4)]) for _ in range(2)]
+        h = np.array([hs[0], hi, hs[1]])
+        u = rng.normal(0, 1, 3)
+        Hb = rng.uniform(-3, 5, 3)
+        cc = rng.uniform(0, 0.05, 3)
+        lim = S.LIM_VANLEER if it % 2 else S.LIM_MINMOD
+        m, c, p = [np.array([h[k], h[k] * u[k], 0.0, Hb[k], h[k] * cc[k]]) for k in range(3)]
+        fm, fp, _ = oracle.cell_faces(lim, m, c, p)
+        assert 0.0 <= fm[0] <= 2 * hi and 0.0 <= fp[0] <= 2 * hi, (h, fm[0], fp[0])
+        assert 0.5 * (fm[0] + fp[0]) == pytest.approx(hi, rel=1e-15, abs=1e-300), (h, fm[0], fp[0])
+        assert fm[4] >= 0.0 and fp[4] >= 0.0
+        clipped += fm[0] == 0.0 or fp[0] == 0.0
+    assert clipped > 100          # the bound was exercised
+
+
+# ----------------------------------------------- suspended-load edge row --
+
+@pytest.mark.parametrize("case", [(2.0, 0.5, 0.1, 1.0, -0.3, 0.2), (1.0, 0.0, 0.0, 3.0, 0.0, 0.0),
+                                  (0.5, 2.5, 0.0, 0.4, 2.2, 1.0), (3.0, -3.0, 0.3, 2.9, -2.5, 0.0),
+                                  (1.2, 0.4, 0.0, 1.0, 0.45, 0.0)])
+def test_edge_suspended_row_is_textbook_hll(oracle, sed, case):
+    """On a flat bed the suspended-load row is the HLL flux of a passive
+    scalar carried with the water, with the same wave speeds:
+    (S_R F_L - S_L F_R + S_L S_R (hc_R - hc_L)) / (S_R - S_L), F = hc u."""
+    p, s = sed
+    s = dict(s, bedload=0)
+    hL, uL, vL, hR, uR, vR = case
+    cL, cR = 0.013, 0.041
+    L = np.array([hL, hL * uL, hL * vL, 0.0, cL * hL])
+    R = np.array([hR, hR * uR, hR * vR, 0.0, cR * hR])
+    dm, dp = oracle.edge_flux(p, s, L, R, (1, 0), 0)
+    sL, sR = math.sqrt(G * hL), math.sqrt(G * hR)
+    ut = (math.sqrt(hL) * uL + math.sqrt(hR) * uR) / (math.sqrt(hL) + math.sqrt(hR))
+    ct = math.sqrt(G * 0.5 * (hL + hR))
+    SL, SR = min(uL - sL, ut - ct), max(uR + sR, ut + ct)
+    FL, FR = cL * hL * uL, cR * hR * uR
+    if SL >= 0:
+        want = FL
+    elif SR <= 0:
+        want = FR
+    else:
+        want = (SR * FL - SL * FR + SL * SR * (cR * hR - cL * hL)) / (SR - SL)
+    assert dm[4] == pytest.approx(want, rel=1e-12, abs=1e-15)
+    assert dp[4] == pytest.approx(-dm[4], rel=1e-14, abs=1e-300)
+    assert dm[3] == 0.0 and dp[3] == 0.0
+
+
+# ----------------------------------------------------- non-erodible mask --
+
+def test_nonerodible_cells_keep_their_bed(oracle):
+    """fixH = 0 marks a non-erodible cell: bedload across every edge of such
+    a cell is zero (SedFlux.cuh:175; interior edges take fixH[p0]*fixH[p1],
+    boundary edges the cell's own flag, VolumeSolver.cu:256,297,313,347,
+    386,402), so with suspension off its bed stays bit-identical while the
+    bed elsewhere moves; with every cell non-erodible nothing moves."""
+    sc = S.gaussian_source(n=40, dx=50.0, seed=2, suspension=0)
+    V = sc.V.copy()
+    V[1] = 0.8 * V[0]            # a current, so bedload runs everywhere wet
+    fix = S.nonerodible_mask(40, 40, 3)
+    fix[:, :3] = 0               # a masked strip along the left wall
+    o = oracle.OracleSim(sc.params, sc.scheme, V, fix)
+    o.step(30)
+    W = o.state()
+    masked = fix == 0
+    assert masked.sum() > 50
+    np.testing.assert_array_equal(W[3][masked], V[3][masked])
+    assert np.abs(W[3] - V[3])[~masked].max() > 1e-6
+    assert abs(W[3].sum() - V[3].sum()) <= 1e-13 * abs(V[3].sum())
+    o2 = oracle.OracleSim(sc.params, sc.scheme, V, np.zeros_like(fix))
+    o2.step(30)
+    np.testing.assert_array_equal(o2.state()[3], V[3])
+
+
+# ----------------------------------------- bed waves (linear Exner theory) --
+
+def _bedload_am(u, Q, Opar, OC):
+    th = Opar * u * u
+    return 17.0 * Q * max(th - OC, 0.0) * max(math.sqrt(th) - math.sqrt(OC), 0.0)
+
+
+def _steady_depth(q, E, H, subcritical):
+    """Depth of steady frictionless flow with discharge q and head E over
+    ground -H: q^2 / (2 h^2) + g (h - H) = E, on the sub- or supercritical
+    branch (Newton from the matching side of the critical depth)."""
+    h = (q * q / G) ** (1.0 / 3.0) * (3.0 if subcritical else 0.3)
+    for _ in range(60):
+        h -= (q * q / (2 * h * h) + G * (h - H) - E) / (G - q * q / h ** 3)
+    return h
+
+
+def _bed_hump_run(oracle, u0, h0, Q, Opar, A=0.05, sig=6.0, n=200, disp=20.0):
+    p = S.base_params(n, 1, 1.0, 1.0, friction=0.0)
+    p.update(Q=Q, Opar=Opar, FR=0.0)
+    s = S.default_scheme(bc=[S.BC_PERIODIC, S.BC_PERIODIC, S.BC_WALL, S.BC_WALL], suspension=0)
+    x = np.arange(n) + 0.5
+    H0 = 10.0
+    bump = A * np.exp(-(x - n / 2) ** 2 / (2 * sig * sig))     # bed raised by the hump
+    H = H0 - bump
+    q = u0 * h0
+    E = 0.5 * u0 * u0 + G * (h0 - H0)
+    fr2 = u0 * u0 / (G * h0)
+    h = np.array([_steady_depth(q, E, Hi, fr2 < 1) for Hi in H])
+    V = S.pack_state(h[None], np.full((1, n), q), np.zeros((1, n)), H[None])
+    o = oracle.OracleSim(p, s, V, np.ones((1, n), np.int32))
+    # linearised Exner celerity under quasi-steady flow:
+    # c_b = q_b'(u0) u0 / (h0 (1 - Fr^2))
+    du = 1e-6 * u0
+    dqb = (_bedload_am(u0 + du, Q, Opar, p["OC"]) - _bedload_am(u0 - du, Q, Opar, p["OC"])) / (2 * du)
+    cb = dqb * u0 / (h0 * (1.0 - fr2))
+    T = abs(disp / cb)
+    while o.t < T:
+        o.step(1)
+    m = H0 - o.state()[3, 0]
+    shift = (x * m).sum() / m.sum() - (x * bump).sum() / bump.sum()
+    return shift, cb * o.t, m, A
+
+
+@pytest.mark.parametrize("regime", ["subcritical", "supercritical"])
+def test_bed_hump_migrates_at_the_exner_celerity(oracle, regime):
+    """Bedload only, steady current over a small bed hump on a periodic
+    channel.  Linear Exner theory: the hump travels at
+    c_b = q_b'(u) u / (h (1 - Fr^2)), downstream in subcritical flow and
+    upstream (antidunes) in supercritical flow.  The bed row of the edge
+    flux is upwinded by the sign of that celerity (VolumeSolver.cu:100-120),
+    so the hump must move the right way at about c_b and stay a hump: no
+    growth, no new trough ahead of or behind it."""
+    if regime == "subcritical":
+        shift, want, m, A = _bed_hump_run(oracle, 1.0, 1.0, 9e-4, 1.0)
+    else:
+        shift, want, m, A = _bed_hump_run(oracle, 4.0, 0.5, 3.5e-5, 1.0)
+    assert (shift > 0) == (regime == "subcritical")
+    assert shift == pytest.approx(want, rel=0.15), (shift, want)
+    assert m.max() <= A
+    assert m.min() >= -0.005 * A, m.min()
+
+
+# ------------------------------------------- suspended load in a current --
+
+def test_suspended_pulse_advects_with_the_flow(oracle):
+    """Suspension and bedload off, uniform current on a flat bed, a
+    concentration plateau: hc is a passive scalar, so its total is
+    conserved, its centroid moves at u t (exactly at first order; the
+    limited slopes' sum shifts it by far less than 0.1 % here), and a TVD
+    reconstruction with an HLL flux makes no new extrema (0 <= c <= c_max).  The concentration faces
+    (Reconstructor.cu:188-194) and the hc row's viscosity
+    (VolumeSolver.cu:98,105) both enter."""
+    n = 256
+    p = S.base_params(n, 1, 2.0, 2.0, friction=0.0)
+    s = S.default_scheme(bc=[S.BC_PERIODIC, S.BC_PERIODIC, S.BC_WALL, S.BC_WALL], suspension=0, bedload=0)
+    x = (np.arange(n) + 0.5) * 2.0
+    h, u = 2.0, 1.5
+    c = np.where(np.abs(x - 150.0) < 30.0, 0.02, 0.0) + 0.01 * np.exp(-((x - 300.0) / 12.0) ** 2)
+    V = S.pack_state(np.full((1, n), h), np.full((1, n), h * u), np.zeros((1, n)), np.full((1, n), 5.0),
+                     (h * c)[None])
+    o = oracle.OracleSim(p, s, V, np.ones((1, n), np.int32))
+    while o.t < 60.0:
+        o.step(1)
+    W = o.state()[:, 0]
+    hc0, hc1 = V[4, 0], W[4]
+    assert abs(hc1.sum() - hc0.sum()) <= 1e-13 * hc0.sum()
+    shift = (x * hc1).sum() / hc1.sum() - (x * hc0).sum() / hc0.sum()
+    assert shift == pytest.approx(u * o.t, rel=1e-3)
+    cn = hc1 / W[0]
+    assert cn.min() >= 0.0 and cn.max() <= 0.02 * (1 + 1e-12), (cn.min(), cn.max())
+    # and the plateau was smeared but not destroyed
+    assert cn.max() > 0.018
+
+
+# ------------------------------------------ exact symmetries of the grid --
+
+def _transpose_state(V):
+    return np.ascontiguousarray(np.stack([V[0].T, V[2].T, V[1].T, V[3].T, V[4].T]))
+
+
+def test_transpose_symmetry_non_square_cells(oracle):
+    """Transposing the grid, swapping qx with qy and dx with dy gives the
+    transposed result bit for bit: the x and y sweeps, the flux divergence
+    (dt/dx, dt/dy) and the cell dt (dx / lambda_x, dy / lambda_y) each use
+    their own axis's spacing."""
+    sc = S.gaussian_source(n=32, dx=60.0, seed=4)
+    pa = dict(sc.params, dy=35.0, ymax=32 * 35.0)
+    pb = dict(sc.params, dx=35.0, xmax=32 * 35.0)
+    V = sc.V.copy()
+    V[1] = 0.3 * V[0]
+    V[2] = -0.2 * V[0]
+    a = oracle.OracleSim(pa, sc.scheme, V, sc.fixH)
+    b = oracle.OracleSim(pb, sc.scheme, _transpose_state(V), np.ascontiguousarray(sc.fixH.T))
+    assert a.dt == b.dt
+    a.step(25)
+    b.step(25)
+    np.testing.assert_array_equal(a.state(), _transpose_state(b.state()).reshape(a.state().shape))
+    assert a.dt == b.dt and a.t == b.t
+
+
+def test_cell_dt_non_square(oracle, sed):
+    p, _ = sed
+    p = dict(p, dx=12.0, dy=5.0)
+    h, qx, qy = 3.0, 1.5, 4.5
+    c = math.sqrt(G * h)
+    want = p["CFL"] * 0.5 * min(12.0 / (0.5 + c), 5.0 / (1.5 + c))
+    assert oracle.cell_dt(p, h, qx, qy) == pytest.approx(want, rel=1e-15)
+    assert oracle.cell_dt(p, h, qy, qx) == pytest.approx(p["CFL"] * 0.5 * min(12.0 / (1.5 + c), 5.0 / (0.5 + c)),
+                                                         rel=1e-15)
+
+
+def test_periodic_shift_invariance(oracle):
+    """On a doubly periodic grid the scheme commutes with a cyclic shift of
+    the cells: neighbour wrap-around (Reconstructor.cu:130-133, 209-212) and
+    the periodic edges (VolumeSolver.cu:217-408) must treat the seam like
+    any interior edge."""
+    n, m = 24, 20
+    rng = np.random.default_rng(21)
+    p = S.base_params(n, m, 15.0, 15.0)
+    s = S.default_scheme(bc=[S.BC_PERIODIC] * 4)
+    x, y = S.cell_centres(n, m, 15.0, 15.0)
+    X, Y = np.meshgrid(x, y)
+    H = 3.0 + 1.5 * np.sin(2 * np.pi * X / (n * 15.0)) * np.cos(2 * np.pi * Y / (m * 15.0)) \
+        + 0.2 * rng.standard_normal((m, n))
+    h = np.maximum(H + 0.6 * np.exp(-((X - 100) ** 2 + (Y - 180) ** 2) / 3000.0), 0.0)
+    V = S.pack_state(h, 0.9 * h, -0.4 * h, H, 2e-3 * h)
+    ones = np.ones((m, n), np.int32)
+    sh = (7, 5)
+    a = oracle.OracleSim(p, s, V, ones)
+    b = oracle.OracleSim(p, s, np.ascontiguousarray(np.roll(V, sh, axis=(1, 2))), ones)
+    a.step(30)
+    b.step(30)
+    np.testing.assert_array_equal(np.roll(a.state(), sh, axis=(1, 2)), b.state())
+    assert np.abs(a.state() - V).max() > 1e-3
+
+
+def test_running_extrema_brute_force(oracle):
+    """The running extrema (Volume_kernel.cu:79-110) after every step: max h
+    and max eta over the steps where h > epsilon_h, max H over all steps;
+    checked against the states read back one step at a time."""
+    sc = S.gaussian_source(n=24, dx=80.0, seed=6, extrema=1)
+    V = sc.V.copy()
+    V[1] = 1.5 * V[0]
+    o = oracle.OracleSim(sc.params, sc.scheme, V, sc.fixH)
+    eh = sc.params["epsilon_h"]
+    hmax = np.full(V[0].shape, -1e30)
+    bmax = np.full(V[0].shape, -1e30)
+    emax = np.full(V[0].shape, -1e30)
+    for _ in range(25):
+        o.step(1)
+        W = o.state()
+        wet = W[0] > eh
+        hmax = np.where(wet, np.maximum(hmax, W[0]), hmax)
+        emax = np.where(wet, np.maximum(emax, W[0] - W[3]), emax)
+        bmax = np.maximum(bmax, W[3])
+    got = o.extrema()
+    np.testing.assert_array_equal(got[0], hmax)
+    np.testing.assert_array_equal(got[1], bmax)
+    np.testing.assert_array_equal(got[2], emax)
+    assert (hmax > -1e30).any() and (hmax == -1e30).any()   # wet and never-wet cells
+
+
+def _inflow_case(side, n=24, m=16, dx=40.0):
+    """A wave entering through one side (0 left, 1 right, 2 bottom, 3 top) of
+    a basin whose depth varies along the entry direction; the CC slots of
+    that side carry (eta(t), u, v, H) as VolumeSolver.cu:172-202 reads them
+    (bottom i, right npx+j, top npx+npy+i, left 2npx+npy+j).  Built for the
+    left side and mapped to the others by mirror and transpose."""
+    x, _ = S.cell_centres(n, m, dx, dx)
+    H = np.broadcast_to((12.0 - 8.0 * x / (n * dx))[None, :], (m, n)).copy()
+    H[5, 9] += 1.5
+    h = np.maximum(H, 0.0)
+    tCC = np.arange(0.0, 400.0 + 1e-9, 20.0)
+    eta = S.n_wave(tCC, 0.8, 200.0)
+    ring = 2 * n + 2 * m
+    CC = np.zeros((len(tCC), ring, 4))
+    CC[:, 2 * n + m:, 0] = eta[:, None]
+    CC[:, 2 * n + m:, 1] = eta[:, None] * np.sqrt(G / H[:, 0])[None, :]
+    CC[:, 2 * n + m:, 2] = 0.05 * eta[:, None]
+    CC[:, 2 * n + m:, 3] = H[:, 0][None, :]
+    V = S.pack_state(h, 0 * h, 0 * h, H, 0 * h)
+    fix = np.ones((m, n), np.int32)
+    bc = [S.BC_INFLOW, S.BC_WALL, S.BC_WALL, S.BC_WALL]
+    p = S.base_params(n, m, dx, dx)
+    if side == 0:
+        return p, bc, V, fix, CC, tCC
+    if side == 1:       # mirror x: left slot j -> right slot npx+j, u -> -u
+        V2 = V[:, :, ::-1].copy()
+        V2[1] *= -1
+        C2 = np.zeros_like(CC)
+        C2[:, n:n + m] = CC[:, 2 * n + m:] * np.array([1, -1, 1, 1.0])
+        return p, [S.BC_WALL, S.BC_INFLOW, S.BC_WALL, S.BC_WALL], np.ascontiguousarray(V2), fix, C2, tCC
+    # transpose: left slot j -> bottom slot j on the n x m grid's transpose
+    Vt = _transpose_state(V)
+    pt = S.base_params(m, n, dx, dx)
+    Ct = np.zeros((len(tCC), ring, 4))
+    Ct[:, 0:m] = CC[:, 2 * n + m:][:, :, [0, 2, 1, 3]]
+    fixt = np.ones((n, m), np.int32)
+    if side == 2:
+        return pt, [S.BC_WALL, S.BC_WALL, S.BC_INFLOW, S.BC_WALL], Vt, fixt, Ct, tCC
+    # side 3, top: mirror the transposed case in y (bottom slot i -> top slot npx+npy+i)
+    V3 = Vt[:, ::-1, :].copy()
+    V3[2] *= -1
+    C3 = np.zeros_like(Ct)
+    C3[:, m + n:2 * m + n] = Ct[:, 0:m] * np.array([1, 1, -1, 1.0])
+    return pt, [S.BC_WALL, S.BC_WALL, S.BC_WALL, S.BC_INFLOW], np.ascontiguousarray(V3), fixt, C3, tCC
+
+
+def test_inflow_slots_of_the_four_sides(oracle):
+    """The same inflow through each side of the basin, with the state
+    mirrored or transposed to match, gives the mirrored or transposed
+    result bit for bit; so each side reads its own slots of the CC ring in
+    the reference's order, with its velocity components in the grid frame."""
+    res = []
+    for side in range(4):
+        p, bc, V, fix, CC, tCC = _inflow_case(side)
+        s = S.default_scheme(bc=bc, suspension=0, bedload=0)
+        o = oracle.OracleSim(p, s, V, fix, CC, tCC)
+        o.step(60)
+        res.append((o.state(), o.t))
+    W0 = res[0][0]
+    assert np.abs(W0[0] - _inflow_case(0)[2][0]).max() > 0.1      # the wave came in
+    W1 = res[1][0][:, :, ::-1].copy()
+    W1[1] *= -1
+    np.testing.assert_array_equal(W1, W0)
+    W2 = _transpose_state(res[2][0])
+    np.testing.assert_array_equal(W2, W0)
+    W3 = res[3][0][:, ::-1, :].copy()
+    W3[2] *= -1
+    np.testing.assert_array_equal(_transpose_state(W3), W0)
+    assert res[0][1] == res[1][1] == res[2][1] == res[3][1]
