@@ -88,3 +88,33 @@ def test_cli_analytic_example_runs(tmp_path):
 def test_cli_missing_input():
     r = subprocess.run([CLI, "/nonexistent/data.dat"], capture_output=True, text=True, timeout=60)
     assert r.returncode != 0 and "not found" in r.stderr
+
+
+@pytest.mark.parametrize("save_frac", [0.0, 0.3, 1.7])
+def test_tsuner_save_interval_around_dt(tmp_path, oracle, save_frac):
+    """Save intervals of zero, shorter than dt and longer than dt: the
+    reference loop saves at most once per pass and then takes one step
+    (GPU/main_TSUNER.cu:157-197), so with an interval shorter than dt every
+    step writes a record of a new state; records and times must equal the
+    oracle loop's."""
+    from scipy.io import netcdf_file
+    from tsuner_amd.solver import run_tsuner
+    n, m, dx = 40, 32, 10.0
+    x, y = S.cell_centres(n, m, dx, dx)
+    X, Y = np.meshgrid(x, y)
+    H = 5.0 - 0.02 * X + 0.3 * np.cos(Y / 40.0)
+    h = np.maximum(0.0, H + 0.8 * np.exp(-((X - 150) ** 2 + (Y - 160) ** 2) / 2000.0))
+    V = S.pack_state(h, 0.3 * h, 0 * h, H)
+    fix = np.ones((m, n), np.int32)
+    dt0 = oracle.OracleSim(S.base_params(n, m, dx, dx), S.default_scheme(), V, fix).dt
+    p = S.base_params(n, m, dx, dx, total_time=12 * dt0, save_time=save_frac * dt0)
+    code, secs = run_tsuner(p, V, fix, str(tmp_path / "run"))
+    assert code == 0
+    recs, times, ext = reference_loop(oracle, p, V, fix, p["total_time"], p["save_time"])
+    with netcdf_file(str(tmp_path / "run.nc"), "r", mmap=False) as f:
+        np.testing.assert_array_equal(f.variables["time"][:], times)
+        assert len(times) == len(recs) and (save_frac > 1 or len(times) >= 8)
+        for k, W in enumerate(recs):
+            np.testing.assert_array_equal(f.variables["eta"][k], W[0] - W[3])
+            np.testing.assert_array_equal(f.variables["ux"][k], W[1] / (W[0] + EPS))
+        np.testing.assert_array_equal(f.variables["eta_max"][:], ext[2])

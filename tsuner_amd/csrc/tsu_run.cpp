@@ -184,10 +184,16 @@ TSU_API int tsu_run(const double* V, const int* fixH, const tsu_sim_params* para
             out.snapshot(rec++, t, host.data());
             next_save += save;
         }
+        // The reference takes exactly one step per pass of its loop, after at
+        // most one snapshot (GPU/main_TSUNER.cu:157-197).  Steps are batched
+        // here up to the next save time; when that time is already due (a
+        // save interval shorter than dt, or zero), the pass is one step, so
+        // every record holds a different state, as in the reference.
+        const bool due = write && save >= 0.0 && next_save <= t;
         double stop = total;
-        if (write && save > 0.0 && next_save < stop) stop = next_save;
+        if (write && save > 0.0 && !due && next_save < stop) stop = next_save;
         long long batch = 1;
-        if (!(write && save == 0.0) && dt > 0.0) {
+        if (!due && dt > 0.0) {
             double est = std::ceil((stop - t) / dt);
             batch = est < 1.0 ? 1 : (est > 512.0 ? 512 : (long long)est);
         }
