@@ -288,7 +288,9 @@ TSU_API int tsu_ensemble_nn_features(tsu_ensemble *e, const float scale_in[8], f
  * takes the depth and discharge corrections if they leave it deeper than
  * shore_h (its bed is unchanged).  resid_max (n floats on the device, or
  * NULL) is raised to the largest |correction of h - H| applied to a wet cell
- * of each member (the fallback indicator). */
+ * of each member (the fallback indicator).  The corrected members' dt for
+ * their next step is recomputed from the corrected state, so that step keeps
+ * to the CFL limit of what it advances. */
 TSU_API int tsu_ensemble_nn_apply(tsu_ensemble *e, const float scale_out[4], float wet_h, const float *out,
                                   long long expect_step, float *resid_max, const void *x, float shore_h);
 /* coarse member i = free-surface restriction of fine member i (fine grids 4x

@@ -7,6 +7,7 @@ import numpy as np
 import pytest
 
 import nn_oracle as O
+import pyoracle
 from tsuner_amd import correction as K
 from tsuner_amd import scenarios as S
 
@@ -111,6 +112,15 @@ def test_features_restrict_target_apply(precision):
                                    atol=1e-6)
         assert np.all(A[1][was_wet & ~wet] == 0) and np.all(A[2][was_wet & ~wet] == 0)
         np.testing.assert_array_equal(A[:, ~was_wet], B[:, ~was_wet])   # dry cells are left alone
+        # the next step's dt is the CFL dt of the corrected state
+        # (GPU/main_TSUNER.cu:147-148 applied to it), not the one the last
+        # stage computed before the correction
+        dtc = min(pyoracle.cell_dt(coarse_sc[i].params, *A[:3, r, c]) for r in range(16) for c in range(16))
+        got_dt = coarse.time(i)[1]
+        if precision == "fp64":
+            assert got_dt == dtc
+        else:
+            assert got_dt == pytest.approx(dtc, rel=1e-5)
     # shoreline mode: with the inputs given, a dry cell next to a wet one (by
     # the inputs' wet flags) takes a depth correction that leaves it deeper
     # than shore_h; other dry cells stay as they are
@@ -134,10 +144,12 @@ def test_features_restrict_target_apply(precision):
         np.testing.assert_array_equal(A[:, dry & ~takes], B[:, dry & ~takes])
     # members whose step count differs are untouched
     snap = [coarse.state(i) for i in range(2)]
+    dts = [coarse.time(i)[1] for i in range(2)]
     coarse.nn_apply(SCALE_OUT, WET, tout.data_ptr(), 4)
     coarse.synchronize()
     for i in range(2):
         np.testing.assert_array_equal(coarse.state(i), snap[i])
+        assert coarse.time(i)[1] == dts[i]
 
 
 def test_ensemble_extrema_match_single_solver():

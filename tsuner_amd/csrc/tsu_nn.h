@@ -54,7 +54,9 @@ hipError_t nn_launch_features(const StageArgs* args, int n, int npx, int npy, in
                               uint16_t* x, hipStream_t s);
 // Adds out * scale to members whose step count equals expect_step (wet
 // cells; with x, the network inputs, also dry shoreline cells); if resid is
-// not null, resid[i] = max(resid[i], largest |correction of h - H|).
+// not null, resid[i] = max(resid[i], largest |correction of h - H|).  The
+// corrected members' dt for the next step is recomputed from the corrected
+// state.
 hipError_t nn_launch_apply(const StageArgs* args, int n, int npx, int npy, int precision, const NnScales& sc,
                            const float* out, long long expect_step, float* resid, const uint16_t* x,
                            hipStream_t s);

@@ -694,6 +694,38 @@ __global__ __launch_bounds__(256) void apply_kernel(const StageArgs* __restrict_
     }
 }
 
+// The correction changes depths and momenta, so the dt the last stage left
+// for the next step (from the uncorrected state) no longer bounds it.  The
+// corrected members' dt slot is re-armed and refilled from the corrected
+// state, as the initial dt is computed (GPU/main_TSUNER.cu:147-148), on the
+// stream and without a host round trip.
+__global__ void apply_dt_reset_kernel(const StageArgs* __restrict__ args, int n, long long expect)
+{
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    Ctrl* C = args[i].ctrl;
+    if (C->step == expect) C->dt_bits[expect & 1] = 0x7FF0000000000000ull;
+}
+
+template <typename S>
+__global__ __launch_bounds__(256) void apply_dt_kernel(const StageArgs* __restrict__ args, int npx, int npy,
+                                                       long long expect)
+{
+    const StageArgs& a = args[blockIdx.y];
+    Ctrl* C = a.ctrl;
+    if (C->step != expect) return;
+    const S* V = (const S*)a.out;
+    const DevParams& P = *a.P;
+    const long long nv = (long long)npx * npy;
+    const double inf = __longlong_as_double(0x7FF0000000000000ll);
+    double m = inf;
+    for (long long p = blockIdx.x * (long long)blockDim.x + threadIdx.x; p < nv; p += (long long)gridDim.x * blockDim.x)
+        m = fmin(m, (double)cell_dt(P, V[p], V[nv + p], V[2 * nv + p]));
+    for (int o = 32; o > 0; o >>= 1) m = fmin(m, __shfl_xor(m, o, 64));
+    if ((threadIdx.x & 63) == 0 && m < inf)
+        atomicMin(&C->dt_bits[expect & 1], (unsigned long long)__double_as_longlong(m));
+}
+
 // Free-surface restriction of the 4x4 block of fine cells under coarse cell
 // (r, c): bed = block mean of H; surface = mean of h - H over the block's wet
 // fine cells (h > kRestrictWetH), depth = max(surface + bed, 0) (0 when no
@@ -799,6 +831,11 @@ hipError_t nn_launch_apply(const StageArgs* args, int n, int npx, int npy, int p
         apply_kernel<double, false><<<g, 256, 0, s>>>(args, npx, npy, sc, out, expect_step, resid, x);
     else
         apply_kernel<float, true><<<g, 256, 0, s>>>(args, npx, npy, sc, out, expect_step, resid, x);
+    apply_dt_reset_kernel<<<(n + 63) / 64, 64, 0, s>>>(args, n, expect_step);
+    if (precision == 0)
+        apply_dt_kernel<double><<<g, 256, 0, s>>>(args, npx, npy, expect_step);
+    else
+        apply_dt_kernel<float><<<g, 256, 0, s>>>(args, npx, npy, expect_step);
     return hipGetLastError();
 }
 
