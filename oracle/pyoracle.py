@@ -103,6 +103,8 @@ def lib():
         L.orc_cell_faces.argtypes = [C.c_int, _dp, _dp, _dp]
         L.orc_shared_quotient_mismatches.restype = C.c_long
         L.orc_shared_quotient_mismatches.argtypes = [C.c_long, C.c_ulonglong]
+        L.orc_shared_quotient_tiny_mismatches.restype = C.c_long
+        L.orc_shared_quotient_tiny_mismatches.argtypes = [C.c_long, C.c_ulonglong, C.c_int]
         _lib = L
     return _lib
 

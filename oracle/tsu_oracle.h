@@ -112,6 +112,9 @@ void   orc_derive_params(orc_sim_params *p);
 /* Test support (tsu_quotient.c): seeded check of the shared-denominator
  * quotient used by the FP64 GPU path against x / d; returns mismatches. */
 long   orc_shared_quotient_mismatches(long n, unsigned long long seed);
+/* The same for numerators from 2^-1074 to 2^-900, with or without the
+ * device's fallback to x / d below 2^-968. */
+long   orc_shared_quotient_tiny_mismatches(long n, unsigned long long seed, int guarded);
 
 #ifdef __cplusplus
 }

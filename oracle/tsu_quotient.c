@@ -58,3 +58,28 @@ long orc_shared_quotient_mismatches(long n, unsigned long long seed)
     return bad;
 }
 
+
+/* The same quotient for tiny numerators: |x| from the smallest subnormal up
+ * to 2^-900 (exponent fields 0..123), d over the solver's denominators.
+ * guarded = 0 counts where q differs from x / d (the residual underflows);
+ * guarded = 1 takes x / d whenever x is nonzero and below 2^-968, as the
+ * device does, and counts the remaining differences. */
+long orc_shared_quotient_tiny_mismatches(long n, unsigned long long seed, int guarded)
+{
+    long bad = 0, i;
+    q_state = seed ? seed : 88172645463325252ull;
+    for (i = 0; i < n; i++) {
+        double d = fabs(q_rand(-30, 30)), x, y, q, r, got;
+        uint64_t bits = ((q_next() % 124) << 52) | (q_next() & ((1ull << 52) - 1)) | (q_next() & (1ull << 63));
+        memcpy(&x, &bits, sizeof x);
+        if (i % 3 == 0) d = fabs(q_rand(-20, 14)) + 2.2204460492503131e-16;
+        if (i % 5 == 0) d = 1.0 + fabs(q_rand(-50, 2));
+        y = 1.0 / d;
+        q = x * y;
+        r = fma(-d, q, x);
+        got = fma(r, y, q);
+        if (guarded && x != 0.0 && fabs(x) < 0x1p-968) got = x / d;
+        if (got != x / d) bad++;
+    }
+    return bad;
+}

@@ -370,7 +370,7 @@ def test_shared_denominator_division_on_the_device():
     equal numpy's correctly rounded 1/a and b/a over [2^-60, 2^60]: random
     denominators, denominators with an all-ones significand (where a
     reciprocal refinement can land on a rounding tie), h + DBL_EPSILON forms
-    and 1 + c forms."""
+    and 1 + c forms; numerators down to subnormals."""
     rng = np.random.default_rng(11)
     sc = S.sediment_shelf(n=8, dx=10.0, inflow=False)
     n = 200000
@@ -383,9 +383,49 @@ def test_shared_denominator_division_on_the_device():
     k = np.arange(n) % 4 == 2
     a[k] = 1.0 + np.abs(rng.normal(size=k.sum())) * 10.0 ** rng.integers(-16, 12, k.sum())
     b = rng.normal(size=n) * 10.0 ** rng.integers(-30, 30, n)
+    # the last quarter (whole waves) takes numerators from the smallest
+    # subnormal to 2^-900, where the residual step can underflow and the
+    # device falls back to the IEEE division
+    t = np.arange(n) >= 3 * n // 4
+    bits = (rng.integers(0, 124, t.sum()).astype(np.uint64) << np.uint64(52)) | \
+        rng.integers(0, 2 ** 52, t.sum(), dtype=np.uint64) | (rng.integers(0, 2, t.sum()).astype(np.uint64) << np.uint64(63))
+    b[t] = bits.view(np.float64)
     y, q, yi, qi = kat_point(sc.params, sc.scheme, 6, a, b)
     np.testing.assert_array_equal(yi, 1.0 / a)                  # the device's IEEE division
     np.testing.assert_array_equal(qi, b / a)
     bad = np.flatnonzero(y != 1.0 / a)
     assert bad.size == 0, "1/a differs at %d inputs, e.g. a=%r: %r vs %r" % (bad.size, a[bad[0]], y[bad[0]], 1.0 / a[bad[0]])
     np.testing.assert_array_equal(q, b / a)
+
+
+def test_fp32_vs_fp64_per_field_sediment_shelf_2048():
+    """FP32 (compensated h/H/hc) against FP64 at the same simulation time on
+    the 2048^2 sediment shelf with its N-wave inflow (1000 steps, both runs
+    ending on the FP64 run's time): per-field errors relative to each
+    field's scale, with tolerances about ten times the measured errors."""
+    import bench
+    sc = S.sediment_shelf(n=2048, dx=10.0, steps=1000)
+    r = bench.precision_error(sc, 1000)
+    for k, v in r["fields"].items():
+        say("fp32 vs fp64 %s: max %.3e rms %.3e (scale %.3g)" % (k, v["max_rel"], v["rms_rel"], v["scale"]))
+    say("totals: water %.3e sediment %.3e" % (r["water_total_rel_diff"], r["sediment_total_rel_diff"]))
+    tol = {"h": 1e-3, "qx": 1e-2, "qy": 1e-2, "H": 1e-3, "hc": 1e-1}
+    for k, v in r["fields"].items():
+        assert v["max_rel"] < tol[k], (k, v)
+    assert abs(r["water_total_rel_diff"]) < 1e-6 and abs(r["sediment_total_rel_diff"]) < 1e-6
+
+
+def test_inflow_series_times_are_checked():
+    """interpolate_time divides by tCC[1] - tCC[0] and truncates to a level
+    index: repeated, decreasing or non-finite times are refused."""
+    sc = S.sediment_shelf(n=32, ny=16, dx=40.0)
+    for bad in ([0.0, 10.0, 10.0, 30.0], [0.0, -5.0, 10.0, 20.0], [0.0, np.nan, 20.0, 30.0],
+                [0.0, 10.0, 20.0, np.inf]):
+        tc = np.array(bad)
+        cc = np.ascontiguousarray(sc.CC[:len(tc)])
+        with pytest.raises(_lib.TsuError) as e:
+            Solver(sc.params, sc.scheme, sc.V, sc.fixH, cc, tc)
+        assert "strictly increasing" in str(e.value)
+    g = Solver.from_scenario(sc)       # the scenario's own series is accepted
+    g.step(3)
+    assert g.steps == 3

@@ -835,3 +835,13 @@ def test_inflow_slots_of_the_four_sides(oracle):
     W3[2] *= -1
     np.testing.assert_array_equal(_transpose_state(W3), W0)
     assert res[0][1] == res[1][1] == res[2][1] == res[3][1]
+
+
+def test_shared_denominator_quotient_tiny_numerators(oracle):
+    """Below 2^-968 the residual step of the shared-denominator quotient can
+    underflow and miss x / d; the device then takes the IEEE division.  On
+    numerators from the smallest subnormal to 2^-900 the unguarded form must
+    miss somewhere (so the threshold matters) and the guarded form never."""
+    L = oracle.lib()
+    assert L.orc_shared_quotient_tiny_mismatches(1_000_000, 777, 0) > 0
+    assert L.orc_shared_quotient_tiny_mismatches(1_000_000, 777, 1) == 0

@@ -486,7 +486,15 @@ TSU_API int tsu_state_set_inflow(tsu_state* st, const double* CC, const double* 
 {
     if (!st || !CC || !tCC) return fail(TSU_ERR_ARG, "null argument");
     if (timeCC < 2) return fail(TSU_ERR_ARG, "the inflow series needs at least two time levels");
+    // interpolate_time divides by tCC[1] - tCC[0] and truncates the quotient
+    // to an int level index (VolumeSolver.cu:174-176): the times must be
+    // finite and increasing
+    for (unsigned long k = 0; k < timeCC; k++)
+        if (!std::isfinite(tCC[k]) || (k > 0 && !(tCC[k] > tCC[k - 1])))
+            return fail(TSU_ERR_ARG, "inflow times must be finite and strictly increasing");
     DeviceGuard g(st->device);
+    // steps already enqueued may still read the old series and run constants
+    HIPCHK(hipStreamSynchronize(st->stream));
     drop_graph(st);
     (void)hipFree(st->CC);
     (void)hipFree(st->tCC);

@@ -201,6 +201,24 @@ __device__ __forceinline__ double qdiv(double x, const Den<double>& D)
 }
 __device__ __forceinline__ float qdiv(float x, const Den<float>& D) { return x * D.y; }
 
+// A nonzero numerator below 2^-968 can make qdiv's residual underflow, and
+// then the corrected product may miss x / d by an ulp (zero is exact).  The
+// solver's quotients of state values (velocities, concentrations, momenta
+// over 1 + c) test their numerators as a group with one wave-wide vote and
+// take the IEEE division for the whole group when any lane has such a
+// numerator (`slow`, uniform over the wave); every other quotient of the
+// solver has a numerator that is zero or far above 2^-968 (products and
+// differences of speeds at least sqrt(g * 2^-1074), or a factor of a
+// quantity that is only used as 1 + c).
+__device__ __forceinline__ bool qtiny(double x) { return x != 0.0 && fabs(x) < 0x1p-968; }
+__device__ __forceinline__ bool wave_any(bool b) { return __builtin_expect(__builtin_amdgcn_ballot_w64(b) != 0, 0); }
+__device__ __forceinline__ double qdiv(double x, const Den<double>& D, bool slow)
+{
+    if (slow) return x / D.d;
+    return qdiv(x, D);
+}
+__device__ __forceinline__ float qdiv(float x, const Den<float>& D, bool) { return x * D.y; }
+
 // SedFlux.cuh:26-49 in FP64.  Divisions with a zero numerator are skipped
 // (0/d is 0); the cap test uses s instead of sqrt(s) (see kVmax2Cut).  D,
 // when given, is den(h + DBL_EPSILON) computed by the caller.
@@ -290,7 +308,8 @@ __device__ __forceinline__ void velocity_conc_pair(double h0, double qx0, double
     const double x0 = qdiv(qx0, D0), y0 = qdiv(qy0, D0);
     const double x1 = qdiv(qx1, D1), y1 = qdiv(qy1, D1);
     if (SED) {
-        const double k0 = qdiv(hc0, D0), k1 = qdiv(hc1, D1);
+        const bool slow = wave_any(qtiny(hc0) || qtiny(hc1));
+        const double k0 = qdiv(hc0, D0, slow), k1 = qdiv(hc1, D1, slow);
         c0 = nc0 ? k0 : 0.0;
         c1 = nc1 ? k1 : 0.0;
     }
@@ -310,7 +329,7 @@ __device__ __forceinline__ void velocity_conc(double h, double qx, double qy, do
     const bool need_c = SED && hc != 0.0;
     if (need_u || need_c) {
         const Den<double> D = den(h + kEpsilon);
-        if (need_c) c = qdiv(hc, D);
+        if (need_c) c = qdiv(hc, D, wave_any(qtiny(hc)));
         u = velocity64(h, qx, qy, &D);
     } else {
         u = velocity64(h, qx, qy);

@@ -864,7 +864,7 @@ __global__ void kat_point_kernel(int what, int law, int n, const double* a, cons
         // beside the IEEE divisions
         const Den<double> D = den(a[i]);
         o1[i] = D.y;
-        o2[i] = qdiv(b[i], D);
+        o2[i] = qdiv(b[i], D, wave_any(qtiny(b[i])));
         o3[i] = 1.0 / a[i];
         o4[i] = b[i] / a[i];
         break;
