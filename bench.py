@@ -166,6 +166,58 @@ def run_small_grids(warmup):
     return out
 
 
+def precision_error(sc, steps):
+    """FP32 (compensated h/H/hc) against FP64 at the same simulation time:
+    the FP64 run takes `steps` steps to time T, the FP32 run steps to T with
+    its crossing step clamped onto T.  Per field (h, qx, qy, H, hc): max and
+    RMS of |FP32 - FP64| over all cells, and both relative to the field's
+    largest FP64 magnitude (momenta: the larger of max |qx|, max |qy|);
+    water and sediment totals of the two runs."""
+    from tsuner_amd.solver import Solver
+    g = Solver.from_scenario(sc, precision="fp64")
+    g.step(steps)
+    T = g.t
+    A = g.state()
+    mA = g.mass()
+    g.close()
+    g = Solver.from_scenario(sc, precision="fp32")
+    g.set_stop_time(T, clamp=True)
+    while g.t < T:
+        g.step(500)
+    steps32 = g.steps
+    B = g.state()
+    mB = g.mass()
+    g.close()
+    if not np.all(np.isfinite(B)):
+        raise RuntimeError("the FP32 run produced non-finite values")
+    qscale = max(np.abs(A[1]).max(), np.abs(A[2]).max())
+    fields = {}
+    for f, name in enumerate(("h", "qx", "qy", "H", "hc")):
+        d = np.abs(B[f] - A[f])
+        scale = qscale if f in (1, 2) else max(np.abs(A[f]).max(), 1e-30)
+        mx, rms = float(d.max()), float(np.sqrt(np.mean(d * d)))
+        fields[name] = {"max_abs": mx, "rms_abs": rms, "scale": float(scale), "max_rel": mx / scale,
+                        "rms_rel": rms / scale}
+    return {"time_s": T, "steps_fp64": steps, "steps_fp32": int(steps32), "fields": fields,
+            "water_total_rel_diff": (mB[0] - mA[0]) / mA[0], "sediment_total_rel_diff": (mB[1] - mA[1]) / mA[1],
+            "units": "h, H, hc in m; qx, qy in m^2/s; relative to the scale given"}
+
+
+def run_fp32_vs_fp64():
+    """The 2048^2 sediment shelf (dx 10 m, fractal bed, non-erodible patches,
+    3 m / 600 s N-wave inflow) for its configured 5000 steps in both
+    precisions, compared field by field at the same simulation time."""
+    from tsuner_amd import scenarios as S
+    sc = S.sediment_shelf(n=2048, dx=10.0, steps=5000)
+    log("[bench] fp32 vs fp64: %s, %d steps" % (sc.name, sc.steps))
+    r = precision_error(sc, sc.steps)
+    r["workload"] = ("sediment shelf 2048x2048, dx=10 m, N-wave inflow (3 m, 600 s), sediment on, %d FP64 steps"
+                     % sc.steps)
+    log("[bench] fp32 vs fp64 at t=%.1f s: " % r["time_s"] + ", ".join(
+        "%s max %.2e rms %.2e" % (k, v["max_rel"], v["rms_rel"]) for k, v in r["fields"].items()))
+    return r
+
+
 ENSEMBLE_SIDES = (256, 384, 512, 768, 1024)
 
 
@@ -361,6 +413,7 @@ def main():
             example = run_reference_example()
         except Exception as e:   # informative leg: reported, never fatal to the headline line
             log("[bench] reference example failed: %r" % (e,))
+    prec_err = None if (args.no_small or args.no_fp32) else run_fp32_vs_fp64()
     config_laws = None
     if not args.no_small:
         # the configurations' own laws where they differ from the reference's:
@@ -422,6 +475,7 @@ def main():
         "active_flow": active,
         "small_grids": small,
         "reference_example": example,
+        "fp32_vs_fp64": prec_err,
         "config_laws": config_laws,
         "ensemble_fine": ensemble,
         "ensemble_coarse_nn": correction,
