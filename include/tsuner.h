@@ -265,6 +265,15 @@ TSU_API int tsu_nn_conv(int kind, const void *in, const void *w, const float *bi
 TSU_API long long tsu_nn_wgrad_workspace(int kind);
 TSU_API int tsu_nn_wgrad(int kind, const void *x, const void *dy, float *dw, float *db, int mrows, int n, int h,
                          int w_, float *workspace, long long workspace_floats, void *stream);
+/* No counterpart in the reference.  tsu_nn_wgrad for callers that cannot
+ * allocate (code under HIP graph capture, training or serving loops that
+ * forbid allocation): the same kernels and result, with the workspace in a
+ * fixed device array of 9,453,568 floats (37.8 MB, the largest kind's need)
+ * that the library holds on each device until the process ends.  No memory is
+ * allocated by the call.  Calls on one device share that array: calls that
+ * overlap on one device (different streams, or threads) race. */
+TSU_API int tsu_nn_wgrad_noalloc(int kind, const void *x, const void *dy, float *dw, float *db, int mrows, int n,
+                                 int h, int w_, void *stream);
 /* dy[p][0..3] = BF16(scale*(out - target)), dy[p][4..7] = 0 for npix pixels
  * of 4 FP32 channels; loss_part[b] (nblocks doubles) = partial sums of the
  * squared error. */

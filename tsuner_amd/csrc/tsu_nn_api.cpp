@@ -5,6 +5,7 @@
 
 #include <hip/hip_runtime.h>
 
+#include <mutex>
 #include <string>
 
 #include "tsu_host.h"
@@ -30,6 +31,44 @@ bool wgrad_shape(int kind, int* my, int* kx)
     case TSU_NN_WGRAD_OUT: *my = 8; *kx = width; return true;
     default: return false;
     }
+}
+
+// Workspace of tsu_nn_wgrad_noalloc: the largest weight-gradient workspace
+// (hidden width 64, the 64 -> 64 layer: per-wave partials of 9 x 64 x 64
+// weights and 64 biases), 9,453,568 floats = 37.8 MB.  Every device the
+// library's code object is loaded on holds one copy until the process ends.
+constexpr long long kScratchFloats = 9LL * tsu::kWgradWaves * 64 * 64 + (long long)tsu::kWgradWaves * 64;
+
+}  // namespace
+
+static __device__ __align__(256) float scratch[kScratchFloats];
+
+namespace {
+
+// Device address of `scratch` per device, looked up once with that device
+// current (the host symbol is not a device address).
+constexpr int kMaxDevices = 64;
+std::mutex g_scratch_mu;
+float* g_scratch_ptr[kMaxDevices] = {};
+
+int scratch_of_device(int dev, float** out)
+{
+    if (dev < 0 || dev >= kMaxDevices) return tsu::set_error(TSU_ERR_ARG, "tsu_nn_wgrad_noalloc: device id out of range");
+    std::lock_guard<std::mutex> lock(g_scratch_mu);
+    if (!g_scratch_ptr[dev]) {
+        int prev = -1;
+        if (hipGetDevice(&prev) != hipSuccess) prev = -1;
+        if (prev != dev && hipSetDevice(dev) != hipSuccess)
+            return tsu::set_error(TSU_ERR_HIP, "tsu_nn_wgrad_noalloc: cannot select the stream's device");
+        void* p = nullptr;
+        const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+        if (prev >= 0 && prev != dev) (void)hipSetDevice(prev);
+        if (e != hipSuccess || !p)
+            return hip_fail("tsu_nn_wgrad_noalloc: hipGetSymbolAddress(scratch)", e != hipSuccess ? e : hipErrorUnknown);
+        g_scratch_ptr[dev] = (float*)p;
+    }
+    *out = g_scratch_ptr[dev];
+    return TSU_OK;
 }
 
 }  // namespace
@@ -72,6 +111,33 @@ TSU_API int tsu_nn_wgrad(int kind, const void* x, const void* dy, float* dw, flo
     hipError_t e = tsu::nn_launch_wgrad(kind, (const uint16_t*)x, (const uint16_t*)dy, part, bpart, dw, db, mrows, n,
                                         h, w_, (hipStream_t)stream);
     return e == hipSuccess ? TSU_OK : hip_fail("tsu_nn_wgrad", e);
+}
+
+TSU_API int tsu_nn_wgrad_noalloc(int kind, const void* x, const void* dy, float* dw, float* db, int mrows, int n,
+                                 int h, int w_, void* stream)
+{
+    int my, kx;
+    if (!wgrad_shape(kind, &my, &kx)) return tsu::set_error(TSU_ERR_ARG, "tsu_nn_wgrad_noalloc: bad kind");
+    if (!x || !dy || !dw || !db || n < 1 || h < 1 || w_ < 1 || mrows < my)
+        return tsu::set_error(TSU_ERR_ARG, "tsu_nn_wgrad_noalloc: bad argument");
+    const long long need = tsu_nn_wgrad_workspace(kind);
+    if (need > kScratchFloats)
+        return tsu::set_error(TSU_ERR_ARG, "tsu_nn_wgrad_noalloc: this kind needs " + std::to_string(need) +
+                                               " workspace floats, more than the " + std::to_string(kScratchFloats) +
+                                               "-float device scratch");
+    int dev = -1;
+    const hipError_t ed = stream ? hipStreamGetDevice((hipStream_t)stream, &dev) : hipGetDevice(&dev);
+    if (ed != hipSuccess) return hip_fail("tsu_nn_wgrad_noalloc: device of the stream", ed);
+    float* ws = nullptr;
+    int rc = scratch_of_device(dev, &ws);
+    if (rc) return rc;
+    // every wave writes all of its partial sums each call: nothing in the
+    // scratch has to start at zero
+    float* part = ws;
+    float* bpart = ws + 9LL * tsu::kWgradWaves * my * kx;
+    hipError_t e = tsu::nn_launch_wgrad(kind, (const uint16_t*)x, (const uint16_t*)dy, part, bpart, dw, db, mrows, n,
+                                        h, w_, (hipStream_t)stream);
+    return e == hipSuccess ? TSU_OK : hip_fail("tsu_nn_wgrad_noalloc", e);
 }
 
 TSU_API int tsu_nn_mse(const float* out, const float* target, void* dy, double* loss_part, int nblocks, long long npix,
