@@ -335,6 +335,18 @@ def run_correction(heldout, fine_steps):
                         "trained_on_pairs": info.get("pairs"), "arch": net.arch.as_dict()}}
 
 
+def calibrate(nbytes, reps=3):
+    """The library's plain copy kernel on nbytes (read and written per
+    launch): GB/s counting both directions, against the HBM peak."""
+    import ctypes as C
+    from tsuner_amd import _lib
+    out = C.c_double()
+    _lib.check(_lib.lib().tsu_bandwidth_probe(0, int(nbytes), int(reps), C.byref(out)), "tsu_bandwidth_probe")
+    log("[bench] copy kernel: %d bytes x %d: %.1f GB/s" % (nbytes, reps, out.value))
+    return {"copy_kernel_gbs": out.value, "bytes_each_way_per_launch": int(nbytes), "launches": reps,
+            "frac_of_peak": out.value / HBM_PEAK_GBS}
+
+
 def cpu_baseline(sc, budget_s=12.0):
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
     import pyoracle as O
@@ -377,7 +389,14 @@ def main():
     ap.add_argument("--no-correction", action="store_true", help="skip the coarse + learned-correction leg")
     ap.add_argument("--no-example", action="store_true", help="skip the reference's data.dat example case")
     ap.add_argument("--correction-scenarios", type=int, default=64)
+    ap.add_argument("--calibrate", nargs="?", const=1e9, type=float, default=None, metavar="BYTES",
+                    help="only run the plain copy kernel (BYTES read and written per launch, 3 launches after a "
+                         "warm-up) and print its bandwidth: the achievable-bandwidth reference and the fixed "
+                         "traffic for calibrating byte counters")
     args = ap.parse_args()
+    if args.calibrate is not None:
+        print(json.dumps(calibrate(int(args.calibrate))), flush=True)
+        return
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
         print("bench.py measures one GPU only: this solver path runs on a single MI355X and no multi-GPU "
               "figure is produced (run with --gpus 1)", file=sys.stderr)

@@ -529,39 +529,44 @@ __device__ __forceinline__ void stage_body(const StageArgs& a, const int tile)
                     out[4 * nv + pos] = (S)W[4];
                 }
             } else {
-                // Compensated accumulation: every flux term enters its pair
-                // exactly; the same product leaves the neighbour across the
-                // edge (Dp = -Dm), so exchanges cancel.
+                // Compensated accumulation in FP64 adds (issued at the FP32
+                // rate): h, H and hc enter as the exact double hi + lo, each
+                // flux term (a float product that leaves the neighbour across
+                // the edge with the opposite sign, Dp = -Dm) is added with one
+                // 53-bit rounding, and the result is stored back as a hi + lo
+                // float pair (48 significant bits), so exchanges cancel to
+                // FP64-level round-off.
                 const int nm = SED ? 3 : 1;
                 const int plane[3] = {0, 3, 4};
-                Pair m[3];
+                double acc[3];
                 for (int j = 0; j < nm; j++) {
                     const int f = plane[j];
-                    m[j].hi = (j == 0) ? R0.h : (j == 1 ? R0.H : R0.hc);
-                    m[j].lo = lo_in[j];
-                    pacc(m[j], -(invx * FL[f]));
-                    pacc(m[j], -(invx * FR[f]));
-                    pacc(m[j], -(invy * FD0[f]));
-                    pacc(m[j], -(invy * FU0[f]));
+                    const R hi = (j == 0) ? R0.h : (j == 1 ? R0.H : R0.hc);
+                    double v = (double)hi + (double)lo_in[j];
+                    v -= (double)(invx * FL[f]);
+                    v -= (double)(invx * FR[f]);
+                    v -= (double)(invy * FD0[f]);
+                    v -= (double)(invy * FU0[f]);
+                    acc[j] = v;
                 }
                 if (SED && P.suspension) {
-                    const R ex = dt * phi;
-                    pacc(m[1], -ex);
-                    pacc(m[2], ex);
+                    const double ex = (double)(dt * phi);
+                    acc[1] -= ex;
+                    acc[2] += ex;
                 }
-                for (int j = 0; j < nm; j++) m[j] = pnorm(m[j]);
                 if (a.k == 1) {
-                    for (int j = 0; j < nm; j++) {
-                        Pair b0;
-                        b0.hi = v0[plane[j]];
-                        b0.lo = lo_v0[j];
-                        m[j] = pavg(b0, m[j]);
-                    }
+                    for (int j = 0; j < nm; j++)
+                        acc[j] = 0.5 * (((double)v0[plane[j]] + (double)lo_v0[j]) + acc[j]);
                     W[1] = (R)0.5 * v0[1] + (R)0.5 * W[1];
                     W[2] = (R)0.5 * v0[2] + (R)0.5 * W[2];
                 }
-                // state filter (SedFlux.cuh:118-132) on the pair values
-                if (m[0].hi < (R)0 || (m[0].hi == (R)0 && m[0].lo <= (R)0)) m[0].hi = m[0].lo = (R)0;
+                // state filter (SedFlux.cuh:118-132) on the accumulated values
+                if (!(acc[0] > 0.0)) acc[0] = 0.0;
+                Pair m[3];
+                for (int j = 0; j < nm; j++) {
+                    m[j].hi = (R)acc[j];
+                    m[j].lo = (R)(acc[j] - (double)m[j].hi);
+                }
                 const R h = m[0].hi;
                 if (h < kNearDry) {
                     R hh = h, hcv = SED ? m[2].hi : (R)0;
@@ -742,11 +747,26 @@ __global__ void warmup_kernel() {}
 
 // Plain streaming copy (16 B per lane per access), the reference point for
 // achievable HBM bandwidth and for calibrating byte counters.
-__global__ __launch_bounds__(256) void copy_kernel(const double2* __restrict__ src, double2* __restrict__ dst,
+// Plain streaming copy of n 16-byte words, the achievable-bandwidth
+// reference: each thread keeps four independent 16-byte loads in flight
+// (a workgroup moves 16 KiB per pass, coalesced 4 KiB at a time), and
+// loads and stores carry the nontemporal hint (the data is touched once).
+typedef unsigned int u32x4 __attribute__((ext_vector_type(4)));
+
+__global__ __launch_bounds__(256) void copy_kernel(const u32x4* __restrict__ src, u32x4* __restrict__ dst,
                                                    long long n)
 {
-    for (long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x; i < n; i += (long long)gridDim.x * blockDim.x)
-        dst[i] = src[i];
+    constexpr int kU = 4;
+    const long long stride = (long long)gridDim.x * blockDim.x * kU;
+    long long i = (long long)blockIdx.x * blockDim.x * kU + threadIdx.x;
+    for (; i + (kU - 1) * (long long)blockDim.x < n; i += stride) {
+        u32x4 v[kU];
+#pragma unroll
+        for (int u = 0; u < kU; u++) v[u] = __builtin_nontemporal_load(src + i + u * (long long)blockDim.x);
+#pragma unroll
+        for (int u = 0; u < kU; u++) __builtin_nontemporal_store(v[u], dst + i + u * (long long)blockDim.x);
+    }
+    for (; i < n; i += blockDim.x) dst[i] = src[i];   // the last partial pass
 }
 
 __global__ void fill_kernel(double* p, long long n, double v)
@@ -937,7 +957,10 @@ hipError_t launch_warmup(hipStream_t s)
 hipError_t launch_copy(const void* src, void* dst, long long bytes, hipStream_t s)
 {
     long long n = bytes / 16;
-    hipLaunchKernelGGL(copy_kernel, dim3(8192), dim3(256), 0, s, (const double2*)src, (double2*)dst, n);
+    // one pass per workgroup where possible, at most 2^16 workgroups
+    long long nb = (n + 4 * 256 - 1) / (4 * 256);
+    const unsigned grid = (unsigned)(nb < 1 ? 1 : (nb > 65536 ? 65536 : nb));
+    hipLaunchKernelGGL(copy_kernel, dim3(grid), dim3(256), 0, s, (const u32x4*)src, (u32x4*)dst, n);
     return hipGetLastError();
 }
 
