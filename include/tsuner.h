@@ -205,7 +205,8 @@ TSU_API void *tsu_device_state(tsu_state *st);
  * and whole-ensemble steps replay from a captured HIP graph.  The reference
  * runs one scenario per TSUNER() call (GPU/main_TSUNER.cu:21-222); members
  * here follow the same arithmetic.  params and scheme are arrays of n; all
- * members share the RK order; inflow series are not supported. */
+ * members share the RK order.  A member with an inflow side takes its own
+ * boundary series (tsu_ensemble_set_inflow) before stepping. */
 typedef struct tsu_ensemble tsu_ensemble;
 
 TSU_API int tsu_ensemble_create(tsu_ensemble **out, int n, const tsu_sim_params *params,
@@ -214,6 +215,10 @@ TSU_API int tsu_ensemble_destroy(tsu_ensemble *e);
 TSU_API int tsu_ensemble_upload(tsu_ensemble *e, int member, const double *V, const int *fixH);
 TSU_API int tsu_ensemble_download(tsu_ensemble *e, int member, double *V);
 TSU_API int tsu_ensemble_init_dt(tsu_ensemble *e);
+/* Boundary series of one member, in the layout of tsu_state_set_inflow
+ * (VolumeSolver.cu:172-202 read per member at that member's time). */
+TSU_API int tsu_ensemble_set_inflow(tsu_ensemble *e, int member, const double *CC, const double *tCC,
+                                    unsigned long timeCC);
 /* Members stop (their steps become no-ops) after stop_step steps. */
 TSU_API int tsu_ensemble_set_stop_step(tsu_ensemble *e, long long stop_step);
 TSU_API int tsu_ensemble_step(tsu_ensemble *e, int nsteps);
@@ -332,7 +337,8 @@ TSU_API int tsu_kat_edge_flux(const tsu_sim_params *params, const tsu_scheme *sc
  * 2 state filter(a,b,c,d) -> o1..o4; 3 cell dt(a,b,c) -> o1;
  * 4 erosion rate(h=a,qx=b,qy=c,hc=d) -> o1; 5 bedload(h=a,un=b,ut=c) -> o1;
  * 6 shared-denominator division: o1 = 1/a, o2 = b/a (a in [2^-60, 2^60]),
- *   o3, o4 the same by IEEE division. */
+ *   o3, o4 the same by IEEE division; 7 square root without operand scaling
+ *   (a = 0 or in [2^-767, DBL_MAX]) -> o1, IEEE sqrt(a) -> o2. */
 TSU_API int tsu_kat_point(const tsu_sim_params *params, const tsu_scheme *scheme, int what, int n,
                           const double *a, const double *b, const double *c, const double *d,
                           double *o1, double *o2, double *o3, double *o4, int device);

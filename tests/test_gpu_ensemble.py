@@ -68,3 +68,55 @@ def test_autotune_keeps_results(tmp_path, monkeypatch):
     for i in range(len(scs)):
         assert np.array_equal(ref.state(i), ens.state(i))
     assert (tmp_path / "tsuner_amd" / "tile_rows.json").exists()
+
+
+def inflow_members():
+    """Members with boundary series of their own: two N-wave shelves of
+    different sizes driven through the left side (different series lengths
+    and spacings), the reference's westport example driven through all four
+    sides, and a walled member beside them."""
+    from ingest_files import westport_scenario
+    a = S.sediment_shelf(n=64, ny=40, dx=40.0, seed=6, t_series=60.0)   # its series ends within the test
+    b = S.sediment_shelf(n=97, ny=33, dx=40.0, seed=7, t_series=2000.0)
+    b.tCC = b.tCC[::2].copy()
+    b.CC = np.ascontiguousarray(b.CC[::2])
+    c = westport_scenario()
+    d = S.gaussian_source(n=50, dx=40.0, seed=8)
+    return [a, b, c, d]
+
+
+def test_fp64_inflow_members_equal_oracle(oracle):
+    """Per-member inflow series (VolumeSolver.cu:172-215 read at each
+    member's own time) in one batched launch: every member equals the
+    oracle bit for bit, through and past the end of its series."""
+    scs = inflow_members()
+    ens = Ensemble(scs, precision="fp64", tile_rows=[8, 5, 16, 4])
+    orc = [oracle.from_scenario(sc) for sc in scs]
+    for chunk in range(3):
+        ens.step(60)
+        for i, (sc, o) in enumerate(zip(scs, orc)):
+            o.step(60)
+            print("chunk %d member %d %s t=%.1f" % (chunk, i, sc.name, o.t), flush=True)
+            assert np.array_equal(ens.state(i), o.state()), "member %d differs from the oracle" % i
+            assert ens.time(i) == (o.t, o.dt, 60 * (chunk + 1))
+
+
+def test_fp32_inflow_members_equal_single_solver():
+    scs = inflow_members()
+    ens = Ensemble(scs, precision="fp32", graph_chunk=10)
+    ens.step(40)
+    for i, sc in enumerate(scs):
+        g = Solver.from_scenario(sc, precision="fp32", tile_rows=16)
+        g.step(40)
+        assert np.array_equal(ens.state(i), g.state()), "member %d" % i
+        assert ens.time(i) == g.time()
+
+
+def test_inflow_member_without_series_is_refused():
+    from tsuner_amd import _lib
+    sc = S.sediment_shelf(n=40, ny=20, dx=40.0)
+    sc.CC = None
+    sc.tCC = None
+    with pytest.raises(_lib.TsuError) as e:
+        Ensemble([sc], precision="fp32")
+    assert "tsu_ensemble_set_inflow" in str(e.value)

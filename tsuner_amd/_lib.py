@@ -105,6 +105,7 @@ _PROTOS = {
     "tsu_ensemble_upload": (C.c_int, [C.c_void_p, C.c_int, _dp, _ip]),
     "tsu_ensemble_download": (C.c_int, [C.c_void_p, C.c_int, _dp]),
     "tsu_ensemble_init_dt": (C.c_int, [C.c_void_p]),
+    "tsu_ensemble_set_inflow": (C.c_int, [C.c_void_p, C.c_int, _dp, _dp, C.c_ulong]),
     "tsu_ensemble_set_stop_step": (C.c_int, [C.c_void_p, C.c_longlong]),
     "tsu_ensemble_step": (C.c_int, [C.c_void_p, C.c_int]),
     "tsu_ensemble_synchronize": (C.c_int, [C.c_void_p]),

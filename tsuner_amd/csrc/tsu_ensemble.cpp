@@ -48,6 +48,7 @@ struct tsu_ensemble {
     std::vector<size_t> xoff;       // member offset in the extrema pool (doubles), if extrema on
     std::vector<int> rows;          // tile rows per member
     std::vector<bool> uploaded;
+    std::vector<double*> cc, tcc;   // per-member inflow series (device), or null
     size_t esize = 8, planes = 5, pool_bytes = 0, fix_bytes = 0;
     char* A = nullptr;
     char* B = nullptr;
@@ -111,6 +112,8 @@ void destroy(tsu_ensemble* e)
     (void)hipFree(e->dmem);
     (void)hipFree(e->tiles);
     (void)hipFree(e->mass_buf);
+    for (double* p : e->cc) (void)hipFree(p);
+    for (double* p : e->tcc) (void)hipFree(p);
     if (e->stream) (void)hipStreamDestroy(e->stream);
     delete e;
 }
@@ -196,6 +199,20 @@ int member_ok(const tsu_ensemble* e, int i)
     return TSU_OK;
 }
 
+// Every member uploaded, and every member with an inflow side given its
+// series.
+int members_ready(const tsu_ensemble* e)
+{
+    for (int i = 0; i < e->n; i++) {
+        if (!e->uploaded[i]) return tsu::set_error(TSU_ERR_ARG, "member " + std::to_string(i) + " not uploaded");
+        for (int sd = 0; sd < 4; sd++)
+            if (e->s[i].bc[sd] == TSU_BC_INFLOW && !e->cc[i])
+                return tsu::set_error(TSU_ERR_ARG, "member " + std::to_string(i) +
+                                                       " has an inflow side: tsu_ensemble_set_inflow first");
+    }
+    return TSU_OK;
+}
+
 int current_buffer(tsu_ensemble* e, int i, char** buf)
 {
     if (e->s[i].order == 2) {
@@ -225,9 +242,6 @@ TSU_API int tsu_ensemble_create(tsu_ensemble** out, int n, const tsu_sim_params*
         if (params[i].npx < 1 || params[i].npy < 1 || !(params[i].dx > 0) || !(params[i].dy > 0))
             return tsu::set_error(TSU_ERR_ARG, "bad grid of member " + std::to_string(i));
         if (scheme[i].order != scheme[0].order) return tsu::set_error(TSU_ERR_UNSUPPORTED, "members must share order");
-        for (int sd = 0; sd < 4; sd++)
-            if (scheme[i].bc[sd] == TSU_BC_INFLOW)
-                return tsu::set_error(TSU_ERR_UNSUPPORTED, "ensemble members do not take inflow series");
     }
     tsu_ensemble* e = new tsu_ensemble();
     e->device = device;
@@ -243,6 +257,8 @@ TSU_API int tsu_ensemble_create(tsu_ensemble** out, int n, const tsu_sim_params*
     e->xoff.assign(n, 0);
     e->rows.assign(n, 16);
     e->uploaded.assign(n, false);
+    e->cc.assign(n, nullptr);
+    e->tcc.assign(n, nullptr);
     long long maxnv = 0;
     for (int i = 0; i < n; i++) {
         e->p[i].nvol = e->p[i].npx * e->p[i].npy;
@@ -373,12 +389,49 @@ TSU_API int tsu_ensemble_download(tsu_ensemble* e, int i, double* V)
     return TSU_OK;
 }
 
+TSU_API int tsu_ensemble_set_inflow(tsu_ensemble* e, int i, const double* CC, const double* tCC, unsigned long timeCC)
+{
+    int rc = member_ok(e, i);
+    if (rc) return rc;
+    if (!CC || !tCC) return tsu::set_error(TSU_ERR_ARG, "null argument");
+    if (timeCC < 2) return tsu::set_error(TSU_ERR_ARG, "the inflow series needs at least two time levels");
+    for (unsigned long k = 0; k < timeCC; k++)
+        if (!std::isfinite(tCC[k]) || (k > 0 && !(tCC[k] > tCC[k - 1])))
+            return tsu::set_error(TSU_ERR_ARG, "inflow times must be finite and strictly increasing");
+    Guard g(e->device);
+    // steps already enqueued may still read the member's old series
+    ECHK(hipStreamSynchronize(e->stream));
+    drop_graph(e);
+    (void)hipFree(e->cc[i]);
+    (void)hipFree(e->tcc[i]);
+    e->cc[i] = e->tcc[i] = nullptr;
+    tsu::DevParams& P = e->P[i];
+    const size_t ncc = (size_t)P.ring * 4 * timeCC;
+    hipError_t err = hipMalloc((void**)&e->cc[i], ncc * sizeof(double));
+    if (err == hipSuccess) err = hipMalloc((void**)&e->tcc[i], timeCC * sizeof(double));
+    if (err != hipSuccess) {
+        (void)hipFree(e->cc[i]);
+        e->cc[i] = nullptr;
+        return tsu::set_error(TSU_ERR_DEVICE_MEMORY, "member inflow series allocation failed");
+    }
+    ECHK(hipMemcpy(e->cc[i], CC, ncc * sizeof(double), hipMemcpyHostToDevice));
+    ECHK(hipMemcpy(e->tcc[i], tCC, timeCC * sizeof(double), hipMemcpyHostToDevice));
+    P.CC = e->cc[i];
+    P.tCC = e->tcc[i];
+    P.timeCC = (long long)timeCC;
+    P.tcc0 = tCC[0];
+    P.dtcc = tCC[1] - tCC[0];
+    P.tcc_end = tCC[timeCC - 2];
+    ECHK(hipMemcpy(e->dP + i, &P, sizeof(tsu::DevParams), hipMemcpyHostToDevice));
+    return TSU_OK;
+}
+
 TSU_API int tsu_ensemble_init_dt(tsu_ensemble* e)
 {
     if (!e) return tsu::set_error(TSU_ERR_ARG, "null ensemble");
     Guard g(e->device);
+    if (int rc = members_ready(e)) return rc;
     for (int i = 0; i < e->n; i++) {
-        if (!e->uploaded[i]) return tsu::set_error(TSU_ERR_ARG, "member " + std::to_string(i) + " not uploaded");
         Ctrl c;
         ECHK(hipMemcpyAsync(&c, e->ctrl + i, sizeof(Ctrl), hipMemcpyDeviceToHost, e->stream));
         ECHK(hipStreamSynchronize(e->stream));
@@ -406,8 +459,7 @@ TSU_API int tsu_ensemble_step(tsu_ensemble* e, int nsteps)
 {
     if (!e) return tsu::set_error(TSU_ERR_ARG, "null ensemble");
     if (nsteps < 0) return tsu::set_error(TSU_ERR_ARG, "negative step count");
-    for (int i = 0; i < e->n; i++)
-        if (!e->uploaded[i]) return tsu::set_error(TSU_ERR_ARG, "member " + std::to_string(i) + " not uploaded");
+    if (int rc = members_ready(e)) return rc;
     Guard g(e->device);
     int done = 0, rc;
     if (e->graph_chunk > 0 && nsteps >= e->graph_chunk) {

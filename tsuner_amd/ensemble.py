@@ -31,6 +31,10 @@ class Ensemble:
             V = np.ascontiguousarray(sc.V, dtype=np.float64)
             fix = np.ascontiguousarray(sc.fixH, dtype=np.int32)
             check(L.tsu_ensemble_upload(self._h, i, ptr(V), ptr(fix, C.c_int)), "tsu_ensemble_upload")
+            if getattr(sc, "CC", None) is not None:
+                cc = np.ascontiguousarray(sc.CC, dtype=np.float64)
+                tc = np.ascontiguousarray(sc.tCC, dtype=np.float64)
+                check(L.tsu_ensemble_set_inflow(self._h, i, ptr(cc), ptr(tc), len(tc)), "tsu_ensemble_set_inflow")
         if tile_rows is not None:
             self.set_tile_rows(tile_rows)
         if graph_chunk:
