@@ -433,3 +433,26 @@ def test_inflow_series_times_are_checked():
     g = Solver.from_scenario(sc)       # the scenario's own series is accepted
     g.step(3)
     assert g.steps == 3
+
+
+def test_unscaled_square_root_on_the_device():
+    """The solver's square root without operand scaling (used where every
+    argument of a wave is 0 or at least 2^-767) equals IEEE sqrt over that
+    range: log-uniform exponents, all-ones and all-zeros significands, exact
+    squares and their neighbours, and the range ends."""
+    rng = np.random.default_rng(17)
+    sc = S.sediment_shelf(n=8, dx=10.0, inflow=False)
+    n = 400000
+    a = np.ldexp(1.0 + rng.random(n), rng.integers(-767, 1023, n))
+    q = np.arange(n) % 8
+    a[q == 1] = np.ldexp(2.0 - 2.0 ** -52, rng.integers(-767, 1023, (q == 1).sum()))
+    r = rng.random((q == 2).sum()) * 1e6 + 1.0
+    a[q == 2] = r * r                                          # exact squares (when representable)
+    a[q == 3] = np.nextafter(a[q == 2][:(q == 3).sum()], np.inf)
+    a[q == 4] = np.nextafter(a[q == 2][:(q == 4).sum()], 0.0)
+    a[q == 5] = rng.random((q == 5).sum()) * 100.0             # depths and speeds of the solver
+    a[:4] = [0.0, 2.0 ** -767, np.nextafter(2.0 ** -767, 1.0), np.finfo(float).max]
+    got, ref = kat_point(sc.params, sc.scheme, 7, a)[:2]
+    np.testing.assert_array_equal(ref, np.sqrt(a))            # the device's sqrt() is the IEEE one
+    bad = np.flatnonzero(got != np.sqrt(a))
+    assert bad.size == 0, "differs at %d inputs, e.g. %r" % (bad.size, a[bad[:3]])

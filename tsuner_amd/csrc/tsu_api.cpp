@@ -126,6 +126,10 @@ void make_dev_params(const tsu_sim_params* p, const tsu_scheme* s, DevParams* P)
     P->order = s->order;
     P->dt_narrow = s->dt_narrow;
     P->extrema = s->extrema;
+    // bedload square roots: |u|^2 > OC (1 - 2^-40) / Opar and O > OC there
+    const double tiny = std::ldexp(1.0, -700);
+    P->bed_sqrt_unscaled = (P->OC > tiny && P->Opar > 0.0 && std::isfinite(P->Opar) &&
+                            P->OC * (1.0 - std::ldexp(1.0, -40)) / P->Opar > tiny) ? 1 : 0;
     for (int i = 0; i < 4; i++) P->bc[i] = s->bc[i];
     P->ring = 2LL * p->npx + 2LL * p->npy;
 }
@@ -798,7 +802,7 @@ TSU_API int tsu_kat_point(const tsu_sim_params* p, const tsu_scheme* s, int what
                           const double* b, const double* c, const double* d, double* o1, double* o2, double* o3,
                           double* o4, int device)
 {
-    if (!p || !s || n < 0 || what < 0 || what > 6) return fail(TSU_ERR_ARG, "bad argument");
+    if (!p || !s || n < 0 || what < 0 || what > 7) return fail(TSU_ERR_ARG, "bad argument");
     if (n == 0) return TSU_OK;
     DevParams P;
     tsu::make_dev_params(p, s, &P);

@@ -65,6 +65,7 @@ struct DevParams {
     double grass_k;         // A_g / (1 - porosity)
     double manning_n;
     int bedload, suspension, limiter, bedload_law, friction_law, order, dt_narrow, extrema;
+    int bed_sqrt_unscaled;  // 1: both bedload square roots have arguments above 2^-700 (see bedload)
     int bc[4];              // left, right, bottom, top
     // boundary inflow series (VolumeSolver.cu:172-202)
     const double* CC;
@@ -159,6 +160,27 @@ struct Vel {
     R x, y;
     R s;   // u.x*u.x + u.y*u.y of the returned (possibly capped) velocity
 };
+
+// The correctly rounded FP64 square root without its operand scaling:
+// rsqrt estimate, one Newton step on (g, h) = (x y, y / 2) and two residual
+// corrections, the compiler's own sequence for sqrt() with the scale
+// factors at 1 (they act only below 2^-767).  Equal to sqrt(x) for x in
+// [2^-767, DBL_MAX]; sqrt_unscaled0 also returns 0 for x = 0.
+__device__ __forceinline__ double sqrt_unscaled(double x)
+{
+    const double y = __builtin_amdgcn_rsq(x);
+    double g = x * y, h = y * 0.5;
+    const double r = fma(-h, g, 0.5);
+    g = fma(g, r, g);
+    h = fma(h, r, h);
+    double d = fma(-g, g, x);
+    g = fma(d, h, g);
+    d = fma(-g, g, x);
+    return fma(d, h, g);
+}
+__device__ __forceinline__ double sqrt_unscaled0(double x) { return x == 0.0 ? x : sqrt_unscaled(x); }
+// x > 0 and below the unscaled square root's range
+__device__ __forceinline__ bool sqrt_needs_scaling(double x) { return x > 0.0 && x < 0x1p-767; }
 
 // |u| as the reference evaluates it after u_calculator (Reconstructor.cu:43,74;
 // SedFlux.cuh:157): sqrt(u.x*u.x + u.y*u.y).
@@ -364,13 +386,34 @@ template <typename R>
 __device__ __forceinline__ R bedload(const DevParams& P, R h, const Vel<R>& u)
 {
     if (!(h > (R)P.HQB && h > (R)P.fiveDS)) return (R)0;
-    R m = speed(u);
-    if (P.bedload_law == BEDLOAD_GRASS) return -((R)P.grass_k * u.x * m * m);
-    R O = (R)P.Opar * m * m;
-    if (!(O > (R)P.OC)) return (R)0;
-    R K1 = rdiv((R)17.0 * u.x, m + (R)kEpsilon);
-    R qb = K1 * (R)P.Q * (O - (R)P.OC) * rmax(rsqrt_(O) - (R)P.sqrtOC, (R)0);
-    return -qb;
+    if (P.bedload_law == BEDLOAD_GRASS) {
+        R m = speed(u);
+        return -((R)P.grass_k * u.x * m * m);
+    }
+    // FP64: the Shields number O = Opar m m with m = sqrt(|u|^2) lies within
+    // a few roundings of Opar |u|^2, so when Opar |u|^2 is below OC by far
+    // more than that (the factor 1 - 2^-40) max(O - OC, 0) is 0 and the
+    // square root is not needed.
+    if constexpr (sizeof(R) == 4) {
+        const R m = speed(u);
+        const R O = (R)P.Opar * m * m;
+        if (!(O > (R)P.OC)) return (R)0;
+        const R K1 = rdiv((R)17.0 * u.x, m + (R)kEpsilon);
+        const R qb = K1 * (R)P.Q * (O - (R)P.OC) * rmax(rsqrt_(O) - (R)P.sqrtOC, (R)0);
+        return -qb;
+    } else {
+        if (!(P.Opar * u.s > P.OC * (1.0 - 0x1p-40))) return 0.0;
+        // Past that test |u|^2 > OC margin / Opar and O > OC; when the run
+        // constants put both above 2^-700 (bed_sqrt_unscaled), the square
+        // roots need no operand scaling.
+        const R m = P.bed_sqrt_unscaled ? sqrt_unscaled(u.s) : speed(u);
+        const R O = (R)P.Opar * m * m;
+        if (!(O > (R)P.OC)) return (R)0;
+        const R K1 = rdiv((R)17.0 * u.x, m + (R)kEpsilon);
+        const R sO = P.bed_sqrt_unscaled ? sqrt_unscaled(O) : rsqrt_(O);
+        const R qb = K1 * (R)P.Q * (O - (R)P.OC) * rmax(sO - (R)P.sqrtOC, (R)0);
+        return -qb;
+    }
 }
 
 // Unrotated (qx, qy) -> normal frame (qn, qt) for the four axis normals
@@ -437,13 +480,28 @@ __device__ __forceinline__ void riemann(const DevParams& P, bool bed_mobile,
 
     // Roe average (SedFlux.cuh:181-189) and HLL speeds (VolumeSolver.cu:63-73)
     R hm = half * (h0 + h1);
-    R sh0 = rsqrt_(h0), sh1 = rsqrt_(h1);
+    R sh0, sh1;
     R SL, SR, cm;
     if constexpr (sizeof(R) == 8) {
-        SL = u0.x - rsqrt_(g * h0);
-        SR = u1.x + rsqrt_(g * h1);
-        cm = rsqrt_(g * hm);
+        // every argument is 0 or at least 2^-767 unless a depth is below
+        // it (g hm >= 4.9 min(h0, h1) when both are wet): then the square
+        // roots skip their scaling, for the whole wave
+        if (!wave_any(sqrt_needs_scaling(h0) || sqrt_needs_scaling(h1))) {
+            sh0 = sqrt_unscaled0(h0);
+            sh1 = sqrt_unscaled0(h1);
+            SL = u0.x - sqrt_unscaled0(g * h0);
+            SR = u1.x + sqrt_unscaled0(g * h1);
+            cm = sqrt_unscaled0(g * hm);
+        } else {
+            sh0 = rsqrt_(h0);
+            sh1 = rsqrt_(h1);
+            SL = u0.x - rsqrt_(g * h0);
+            SR = u1.x + rsqrt_(g * h1);
+            cm = rsqrt_(g * hm);
+        }
     } else {
+        sh0 = rsqrt_(h0);
+        sh1 = rsqrt_(h1);
         // FP32 solver: sqrt(g h) as sqrt(g) sqrt(h), reusing the Roe weights
         const R sg = (R)kSqrtG;
         SL = u0.x - sg * sh0;

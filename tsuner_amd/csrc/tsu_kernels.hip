@@ -480,7 +480,10 @@ __device__ __forceinline__ void stage_body(const StageArgs& a, const int tile)
             } else {
                 un = velocity64(v0[0], v0[1], v0[2], &Dh);
             }
-            const R modu = speed(un);
+            // |u| of U^n (friction, exchange); in FP64 a wave at rest skips
+            // the square root of 0
+            R modu = (R)0;
+            if (COMP || __builtin_amdgcn_ballot_w64(un.s != (R)0) != 0) modu = speed(un);
             R cf;
             if (P.friction_law == FRIC_MANNING) {
                 R hp = v0[0] + (R)kEpsilon;
@@ -889,6 +892,11 @@ __global__ void kat_point_kernel(int what, int law, int n, const double* a, cons
         o4[i] = b[i] / a[i];
         break;
     }
+    case 7:
+        // the square root without operand scaling beside sqrt()
+        o1[i] = sqrt_unscaled0(a[i]);
+        o2[i] = sqrt(a[i]);
+        break;
     }
 }
 
