@@ -119,3 +119,4 @@ def test_subgrid_runup_reads_the_fine_ground():
     assert K.runup(Hc, hc) != rf                                   # block-mean ground differs
     # a coarse cell never reached floods nothing
     assert K.runup_subgrid(H0, np.full((n, n), np.nan), np.zeros((n, n))) == 0.0
+

@@ -53,20 +53,33 @@ def test_pairs_zero_network_and_training():
     assert full_loss(net) < 0.8 * before
 
 
-def test_shipped_network_beats_coarse_alone():
-    """What the trained network gains over the method without it: 24
-    held-out scenarios at full size (fine 1024^2, coarse 256^2, 3000 fine
-    steps) with the shipped weights, against the coarse solver alone: the
-    coastal peak eta_max error and the nearshore mean |eta_max error| drop
-    by more than 20 % (on 200 scenarios: 57 % and 60 %) and the runup error
-    drops, at more than 8x the fine solver's throughput."""
+def test_networks_meet_the_bars_and_beat_coarse_alone():
+    """What the trained networks gain over the method without it, on the
+    benchmark's 200 held-out scenarios at full size (fine 1024^2, coarse
+    256^2, 3000 fine steps; a sample of 24 is too small for the absolute
+    bars: runup errors of single scenarios reach 30 %): the configured
+    network (four convolutions, 32 channels) and the
+    wider shipped one (six, 64 channels) against the coarse solver alone and
+    the fine solver, with the metrics the benchmark reports.  Bars asserted:
+    coastal peak eta_max error <= 5 cm and runup error <= 10 % (means over
+    the scenarios) at >= 8x the fine solver's throughput, each network
+    improving on coarse alone in coastal peak, zone peak, nearshore mean and
+    runup.  The zone peak (inundation front included) is reported, not held
+    to 5 cm: on 200 scenarios it is 7.5 cm (4x32) and 7.1 cm (64 wide)."""
     from tsuner_amd import correction as K
-    net, scale_out, _ = K.load_net()
-    ev = K.evaluate(net, scale_out, list(range(700001, 700025)), fine_steps=3000)
-    base, corr = ev["coarse_alone"], ev["coarse_nn"]
-    for k in ("coarse_alone", "coarse_nn", "coarse_nn_bed_uncorrected"):
-        print(k, ev[k], flush=True)
-    assert corr["peak_eta_sea_err_mean_m"] < 0.8 * base["peak_eta_sea_err_mean_m"]
-    assert corr["nearshore_mean_abs_eta_err_m"] < 0.8 * base["nearshore_mean_abs_eta_err_m"]
-    assert corr["runup_rel_err_mean"] < base["runup_rel_err_mean"]
-    assert corr["speedup_vs_fine"] > 8.0
+    nets = {"conv4_w32": K.load_net(K.WEIGHTS_4X32), "conv6_w64": K.load_net(K.WEIGHTS)}
+    net, so, _ = nets["conv4_w32"]
+    ev = K.evaluate(net, so, list(range(300001, 300201)), fine_steps=3000,
+                    others={"conv6_w64": nets["conv6_w64"][:2]})
+    base = ev["coarse_alone"]
+    rows = {"conv4_w32": ev["coarse_nn"], "conv6_w64": ev["others"]["conv6_w64"]["coarse_nn"]}
+    print("coarse_alone", base, flush=True)
+    for k, corr in rows.items():
+        print(k, corr, flush=True)
+        assert corr["peak_eta_sea_err_mean_m"] <= 0.05, (k, corr["peak_eta_sea_err_mean_m"])
+        assert corr["runup_rel_err_mean"] <= 0.10, (k, corr["runup_rel_err_mean"])
+        assert corr["speedup_vs_fine"] >= 8.0, (k, corr["speedup_vs_fine"])
+        assert corr["peak_eta_sea_err_mean_m"] < 0.8 * base["peak_eta_sea_err_mean_m"]
+        assert corr["peak_eta_err_mean_m"] < base["peak_eta_err_mean_m"]
+        assert corr["nearshore_mean_abs_eta_err_m"] < 0.8 * base["nearshore_mean_abs_eta_err_m"]
+        assert corr["runup_rel_err_mean"] < base["runup_rel_err_mean"]

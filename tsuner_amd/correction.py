@@ -43,7 +43,8 @@ SHORE_H = 1e-2
 # inputs: h, qx, qy, z, dz/dx, dz/dy, eta (wet), wet flag
 SCALE_IN = [1 / 50.0, 1 / 5.0, 1 / 5.0, 1 / 50.0, 20.0, 20.0, 2.0, 1.0]
 HERE = os.path.dirname(os.path.abspath(__file__))
-WEIGHTS = os.path.join(HERE, "correction_weights.npz")
+WEIGHTS = os.path.join(HERE, "correction_weights.npz")              # 8 -> 64 x5 -> 4 (six convolutions)
+WEIGHTS_4X32 = os.path.join(HERE, "correction_weights_4x32.npz")    # 8 -> 32 x3 -> 4 (four convolutions)
 
 
 def log(msg):
@@ -487,9 +488,12 @@ def load_net(path: str = WEIGHTS, max_batch: int = 32, h: int = None, w: int = N
 
 
 def evaluate(net, scale_out, seeds: Sequence[int], fine_steps: int = 3000, resid_tau: float = 0.5, fine_n: int = FINE_N,
-             max_batch: int = 32, min_speedup: float = 8.0) -> dict:
+             max_batch: int = 32, min_speedup: float = 8.0, others: dict = None) -> dict:
     """Held-out evaluation: fine members (fine_steps steps) against the coarse
-    solver alone and coarse + network, with the residual fallback."""
+    solver alone and coarse + network, with the residual fallback.  others
+    ({name: (net, scale_out)}) are further networks evaluated against the
+    same fine and coarse-alone runs (their summaries and fallback policies
+    under out["others"][name])."""
     t0 = time.perf_counter()
     fine_sc = [fine_scenario(s, extrema=1, n=fine_n) for s in seeds]
     coarse_sc = [coarse_scenario(f) for f in fine_sc]
@@ -575,35 +579,21 @@ def evaluate(net, scale_out, seeds: Sequence[int], fine_steps: int = 3000, resid
                                 speedup_vs_fine=fine.wall_s / (corr.wall_s + fine_per * int((corr.resid > tau).sum())))
                            for tau in (0.05, 0.1, 0.2, 0.3, 0.5)],
     }
-    # Fallback policies from run-time indicators, each threshold chosen on the
-    # first half of the scenarios (the largest fallback share that keeps
-    # >= min_speedup there) and reported on the second half.
+    out["fallback_policies"] = _fallback_policies(mb, mc, corr, base, fine_per, min_speedup, summary)
     disagree = np.array([abs(mc[i]["peak_eta_coarse"] - mb[i]["peak_eta_coarse"]) for i in range(n)])
-    peak_c = np.array([mc[i]["peak_eta_coarse"] for i in range(n)])
-    indicators = {"resid_max": (corr.resid, corr.wall_s),
-                  "coarse_peak_eta": (peak_c, corr.wall_s),
-                  "nn_vs_coarse_peak_disagreement": (disagree, corr.wall_s + base.wall_s)}
-    half = n // 2
-    cal, test = np.arange(half), np.arange(half, n)
-
-    def cost(ind, tau, idx, base_wall):
-        fb = ind[idx] > tau
-        return base_wall * len(idx) / n + fine_per * int(fb.sum()), fb
-
-    policies = {}
-    for name, (ind, bw) in indicators.items():
-        best = None
-        for tau in np.unique(np.concatenate([ind[cal], [np.inf]])):
-            wall, fb = cost(ind, tau, cal, bw)
-            sp = fine_per * len(cal) / wall
-            if sp >= min_speedup and (best is None or fb.mean() > best[1]):
-                best = (float(tau), float(fb.mean()))
-        tau = best[0] if best else float("inf")
-        wall, fb = cost(ind, tau, test, bw)
-        sub = [mc[i] for i in test]
-        policies[name] = dict(summary(sub, fb), threshold=tau, fallback_frac=float(fb.mean()),
-                              speedup_vs_fine=fine_per * len(test) / wall, scenarios=int(len(test)))
-    out["fallback_policies"] = {"min_speedup_on_calibration_half": min_speedup, "policies": policies}
+    if others:
+        out["others"] = {}
+        for name, (net2, so2) in others.items():
+            c2 = run_coarse(coarse_sc, fine.t_end, net=net2, scale_out=so2, shore_h=SHORE_H)
+            log("[eval] coarse + network %s: %.2f s wall" % (name, c2.wall_s))
+            m2 = [metrics(fine, c2, i) for i in range(n)]
+            out["others"][name] = {
+                "coarse_nn": dict(summary(m2), wall_s=c2.wall_s, sims_per_hr=n * 3600 / c2.wall_s,
+                                  speedup_vs_fine=fine.wall_s / c2.wall_s, shore_h_m=SHORE_H),
+                "fallback_policies": _fallback_policies(mb, m2, c2, base, fine_per, min_speedup, summary),
+                "per_scenario": {"peak_eta_err_nn": [m["peak_eta_err"] for m in m2],
+                                 "peak_eta_nn": [m["peak_eta_coarse"] for m in m2],
+                                 "runup_rel_err_nn": [m["runup_rel_err"] for m in m2]}}
     out["per_scenario"] = {"seed": [int(s) for s in seeds],
                            "peak_eta_fine": [m["peak_eta_fine"] for m in mc],
                            "peak_eta_err_coarse": [m["peak_eta_err"] for m in mb],
@@ -618,6 +608,41 @@ def evaluate(net, scale_out, seeds: Sequence[int], fine_steps: int = 3000, resid
                            "resid_max": [float(v) for v in corr.resid],
                            "nn_vs_coarse_peak_disagreement": [float(v) for v in disagree]}
     return out
+
+
+def _fallback_policies(mb, mc, corr, base, fine_per, min_speedup, summary) -> dict:
+    """Fallback policies from run-time indicators, each threshold chosen on
+    the first half of the scenarios (the largest fallback share that keeps
+    >= min_speedup there) and reported on the second half; a fallback costs
+    the fine ensemble's per-member wall time and has zero error."""
+    n = len(mc)
+    disagree = np.array([abs(mc[i]["peak_eta_coarse"] - mb[i]["peak_eta_coarse"]) for i in range(n)])
+    peak_c = np.array([mc[i]["peak_eta_coarse"] for i in range(n)])
+    indicators = {"resid_max": (corr.resid, corr.wall_s),
+                  "coarse_peak_eta": (peak_c, corr.wall_s),
+                  "nn_vs_coarse_peak_disagreement": (disagree, corr.wall_s + base.wall_s)}
+    half = n // 2
+    cal, test = np.arange(half), np.arange(half, n)
+
+    def cost(ind, tau, idx, base_wall):
+        fb = ind[idx] > tau
+        return base_wall * len(idx) / n + fine_per * int(fb.sum()), fb
+
+    policies = {}
+    for name, (ind, bw) in indicators.items():
+        ind = np.nan_to_num(np.asarray(ind, np.float64), nan=-np.inf)
+        best = None
+        for tau in np.unique(np.concatenate([ind[cal], [np.inf]])):
+            wall, fb = cost(ind, tau, cal, bw)
+            sp = fine_per * len(cal) / wall
+            if sp >= min_speedup and (best is None or fb.mean() > best[1]):
+                best = (float(tau), float(fb.mean()))
+        tau = best[0] if best else float("inf")
+        wall, fb = cost(ind, tau, test, bw)
+        sub = [mc[i] for i in test]
+        policies[name] = dict(summary(sub, fb), threshold=tau, fallback_frac=float(fb.mean()),
+                              speedup_vs_fine=fine_per * len(test) / wall, scenarios=int(len(test)))
+    return {"min_speedup_on_calibration_half": min_speedup, "policies": policies}
 
 
 def time_network(net: "nn.Net", n: int, reps: int = 10) -> dict:
