@@ -127,6 +127,17 @@ __device__ __forceinline__ int sgn(R x)
 template <typename R>
 __device__ __forceinline__ R lim_vanleer(R a, R b)
 {
+    if constexpr (sizeof(R) == 4) {
+        // FP32 solver: for slopes of one strict sign |a| + |b| = |a + b| and
+        // the quotient is 2ab / (a + b); ab > 0 selects exactly those pairs
+        // (a product that underflows gives 0 where the reference's value,
+        // at most 2 min(|a|, |b|) <= 2 sqrt(ab), is below 1e-22), and then
+        // |a + b| >= 2 sqrt(ab) > 0, so the reciprocal is finite.
+        const float p = a * b;
+        const bool use = p > 0.f;
+        const float r = __builtin_amdgcn_rcpf(use ? a + b : 1.f);
+        return use ? (p + p) * r : 0.f;
+    }
     R aa = rabs(a), bb = rabs(b);
     R s = aa + bb;
     const bool use = ((a > (R)0 && b > (R)0) || (a < (R)0 && b < (R)0)) && s > (R)kEpsilon;
@@ -361,8 +372,8 @@ __device__ __forceinline__ void velocity_conc(double h, double qx, double qy, do
 template <bool SED>
 __device__ __forceinline__ void velocity_conc(float h, float qx, float qy, float hc, Vel<float>& u, float& c)
 {
-    c = 0.f;
-    if (SED && hc != 0.f) c = rdiv(hc, h + (float)kEpsilon);
+    // evaluated for every lane and selected (no branch)
+    c = (SED && hc != 0.f) ? hc * __builtin_amdgcn_rcpf(h + (float)kEpsilon) : 0.f;
     u = velocity(h, qx, qy);
 }
 
@@ -443,9 +454,12 @@ __device__ __forceinline__ void from_normal(int kind, R a, R b, R& fx, R& fy)
 
 // VolumeSolver.cu:19-126 in the normal frame: hydrostatic reconstruction and
 // the HLL/PVM path-conservative fluctuations Dm (to the left cell) and Dp (to
-// the right cell).  Inputs: left/right face (h, H, qn, qt, hc).  With SED
-// false the bed and concentration rows are identically zero (no bedload, no
-// suspension, hc == 0) and are not computed.
+// the right cell).  Inputs: left/right face (h, H, qn, qt, hc) in FP64; in
+// the FP32 solver the faces carry (h, H, u_n, u_t, c), the velocity and
+// concentration that the state filter (or the cell's own velocity, first
+// order) already computed, which the FP64 path recomputes from h u and h c.
+// With SED false the bed and concentration rows are identically zero (no
+// bedload, no suspension, hc == 0) and are not computed.
 template <typename R, bool SED>
 __device__ __forceinline__ void riemann(const DevParams& P, bool bed_mobile,
                                         R hl, R zl, R qnl, R qtl, R hcl,
@@ -468,8 +482,10 @@ __device__ __forceinline__ void riemann(const DevParams& P, bool bed_mobile,
     if constexpr (sizeof(R) == 8) {
         velocity_conc_pair<SED>(hl, qnl, qtl, hcl, hr, qnr, qtr, hcr, u0, c0, u1, c1);
     } else {
-        velocity_conc<SED>(hl, qnl, qtl, hcl, u0, c0);
-        velocity_conc<SED>(hr, qnr, qtr, hcr, u1, c1);
+        u0 = {qnl, qtl, qnl * qnl + qtl * qtl};
+        u1 = {qnr, qtr, qnr * qnr + qtr * qtr};
+        c0 = SED ? hcl : zero;
+        c1 = SED ? hcr : zero;
     }
     R q0n = u0.x * h0, q0t = u0.y * h0;
     R q1n = u1.x * h1, q1t = u1.y * h1;
@@ -672,6 +688,11 @@ __device__ __forceinline__ void ghost_state(const DevParams& P, int side, const 
             double gd[5];
             inflow_state(P, ct, idx, gd);
             for (int f = 0; f < 5; f++) g[f] = (R)gd[f];
+            if constexpr (sizeof(R) == 4) {   // FP32 faces carry velocities
+                const Vel<float> u = velocity(g[0], g[1], g[2]);
+                g[1] = u.x;
+                g[2] = u.y;
+            }
         } else {
             g[ax] = -in[ax];
         }

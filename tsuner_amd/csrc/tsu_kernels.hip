@@ -190,10 +190,24 @@ __device__ __forceinline__ R recons_axis(int lim, const Cell<R>& ci, R hl, R Hl,
     return deta;
 }
 
+// Faces after the state filter (SedFlux.cuh:118-132).  FP64 keeps them as
+// (h, qx, qy, H, hc) = (h, h u, h u, H, h c); the FP32 solver keeps the
+// filter's (h, ux, uy, H, c), which is what the Riemann solver reads.
 template <typename R, bool SED>
 __device__ __forceinline__ void filter_face(R* F)
 {
-    state_filter<R, SED>(F[0], F[1], F[2], F[4]);
+    if constexpr (sizeof(R) == 4) {
+        const float h = (F[0] > 0.f) ? F[0] : 0.f;
+        Vel<float> u;
+        float c;
+        velocity_conc<SED>(h, F[1], F[2], F[4], u, c);
+        F[0] = h;
+        F[1] = u.x;
+        F[2] = u.y;
+        F[4] = (SED && c > 0.f) ? c : 0.f;
+    } else {
+        state_filter<R, SED>(F[0], F[1], F[2], F[4]);
+    }
 }
 
 // The two faces of a cell filtered together (values of filter_face on each);
@@ -223,14 +237,15 @@ __device__ __forceinline__ void filter_faces(R* A, R* B)
     }
 }
 
+// First-order faces: the cell state, as filtered faces are kept.
 template <typename R>
 __device__ __forceinline__ void cell_face(const Cell<R>& c, R* F)
 {
     F[0] = c.h;
-    F[1] = c.qx;
-    F[2] = c.qy;
+    F[1] = (sizeof(R) == 4) ? c.ux : c.qx;
+    F[2] = (sizeof(R) == 4) ? c.uy : c.qy;
     F[3] = c.H;
-    F[4] = c.hc;
+    F[4] = (sizeof(R) == 4) ? c.c : c.hc;
 }
 
 // Riemann problem on one edge with left/right faces in the unrotated frame;
