@@ -124,8 +124,11 @@ __device__ __forceinline__ int sgn(R x)
 // division is only evaluated for same-sign slopes; there a|b| and b|a| are
 // both +-RN(ab) (rounding is sign-symmetric), so the numerator is
 // +-2 RN(ab) exactly.
+// In two parts: whether the quotient is taken (vl_use) and the limited
+// slope given that flag (vl_val), so that callers can skip the quotients of
+// a set of slopes with one wave-wide vote.
 template <typename R>
-__device__ __forceinline__ R lim_vanleer(R a, R b)
+__device__ __forceinline__ bool vl_use(R a, R b)
 {
     if constexpr (sizeof(R) == 4) {
         // FP32 solver: for slopes of one strict sign |a| + |b| = |a + b| and
@@ -133,19 +136,34 @@ __device__ __forceinline__ R lim_vanleer(R a, R b)
         // (a product that underflows gives 0 where the reference's value,
         // at most 2 min(|a|, |b|) <= 2 sqrt(ab), is below 1e-22), and then
         // |a + b| >= 2 sqrt(ab) > 0, so the reciprocal is finite.
-        const float p = a * b;
-        const bool use = p > 0.f;
+        return a * b > 0.f;
+    } else {
+        const R s = rabs(a) + rabs(b);
+        return ((a > (R)0 && b > (R)0) || (a < (R)0 && b < (R)0)) && s > (R)kEpsilon;
+    }
+}
+
+template <typename R>
+__device__ __forceinline__ R vl_val(R a, R b, bool use)
+{
+    const R p = a * b;   // > 0 for same-sign slopes
+    if constexpr (sizeof(R) == 4) {
         const float r = __builtin_amdgcn_rcpf(use ? a + b : 1.f);
         return use ? (p + p) * r : 0.f;
+    } else {
+        // evaluated for every lane and selected, so the limiters of a face
+        // set share one basic block; the divisor is 1 where the result is
+        // unused
+        const R s = rabs(a) + rabs(b);
+        const R d = rdiv(a > (R)0 ? p + p : -(p + p), use ? s : (R)1);
+        return use ? d : (R)0;
     }
-    R aa = rabs(a), bb = rabs(b);
-    R s = aa + bb;
-    const bool use = ((a > (R)0 && b > (R)0) || (a < (R)0 && b < (R)0)) && s > (R)kEpsilon;
-    const R p = a * b;                      // > 0 for same-sign slopes
-    // evaluated for every lane and selected, so the limiters of a face set
-    // share one basic block; the divisor is 1 where the result is unused
-    const R d = rdiv(a > (R)0 ? p + p : -(p + p), use ? s : (R)1);
-    return use ? d : (R)0;
+}
+
+template <typename R>
+__device__ __forceinline__ R lim_vanleer(R a, R b)
+{
+    return vl_val(a, b, vl_use(a, b));
 }
 
 template <typename R>
@@ -279,8 +297,10 @@ __device__ __forceinline__ Vel<double> velocity64(double h, double qx, double qy
 __device__ __forceinline__ Vel<double> velocity(double h, double qx, double qy) { return velocity64(h, qx, qy); }
 
 // FP32 solver: near-dry cells take the reference's FP64 desingularisation,
-// wet cells a single reciprocal.
-__device__ __forceinline__ Vel<float> velocity(float h, float qx, float qy)
+// wet cells a single reciprocal.  rh, when given, is rcp(h + DBL_EPSILON)
+// computed by the caller: for h >= 1 cm, h + DBL_EPSILON rounds to h in FP32,
+// so it is the wet path's rcp(h).
+__device__ __forceinline__ Vel<float> velocity(float h, float qx, float qy, const float* rh = nullptr)
 {
     Vel<float> u;
     if (h < kNearDry) {
@@ -290,7 +310,7 @@ __device__ __forceinline__ Vel<float> velocity(float h, float qx, float qy)
         u.s = (float)d.s;
         return u;
     }
-    float r = __builtin_amdgcn_rcpf(h);
+    const float r = rh ? *rh : __builtin_amdgcn_rcpf(h);
     u.x = qx * r;
     u.y = qy * r;
     float s = u.x * u.x + u.y * u.y;
@@ -372,9 +392,11 @@ __device__ __forceinline__ void velocity_conc(double h, double qx, double qy, do
 template <bool SED>
 __device__ __forceinline__ void velocity_conc(float h, float qx, float qy, float hc, Vel<float>& u, float& c)
 {
-    // evaluated for every lane and selected (no branch)
-    c = (SED && hc != 0.f) ? hc * __builtin_amdgcn_rcpf(h + (float)kEpsilon) : 0.f;
-    u = velocity(h, qx, qy);
+    // one reciprocal for the concentration and the velocity, evaluated for
+    // every lane and selected (no branch)
+    const float r = __builtin_amdgcn_rcpf(h + (float)kEpsilon);
+    c = (SED && hc != 0.f) ? hc * r : 0.f;
+    u = velocity(h, qx, qy, &r);
 }
 
 // SedFlux.cuh:118-132
@@ -406,9 +428,11 @@ __device__ __forceinline__ R bedload(const DevParams& P, R h, const Vel<R>& u)
     // more than that (the factor 1 - 2^-40) max(O - OC, 0) is 0 and the
     // square root is not needed.
     if constexpr (sizeof(R) == 4) {
-        const R m = speed(u);
-        const R O = (R)P.Opar * m * m;
+        // FP32 solver: O from |u|^2 directly, the square root only past the
+        // threshold
+        const R O = (R)P.Opar * u.s;
         if (!(O > (R)P.OC)) return (R)0;
+        const R m = speed(u);
         const R K1 = rdiv((R)17.0 * u.x, m + (R)kEpsilon);
         const R qb = K1 * (R)P.Q * (O - (R)P.OC) * rmax(rsqrt_(O) - (R)P.sqrtOC, (R)0);
         return -qb;

@@ -162,28 +162,47 @@ __device__ __forceinline__ R recons_axis(int lim, const Cell<R>& ci, R hl, R Hl,
     Fp[0] = hi + half * dh;
 
     R zm = rmin(Hl, Hi);
-    dl = rmax(hi - Hi + zm, zero) - rmax(hl - Hl + zm, zero);
+    const R dle = rmax(hi - Hi + zm, zero) - rmax(hl - Hl + zm, zero);
     zm = rmin(Hr, Hi);
-    dr = rmax(hr - Hr + zm, zero) - rmax(hi - Hi + zm, zero);
-    R deta = limiter(lim, dl, dr);
+    const R dre = rmax(hr - Hr + zm, zero) - rmax(hi - Hi + zm, zero);
+    const R dlu = ci.ux - ulx, dru = urx - ci.ux;
+    const R dlv = ci.uy - uly, drv = ury - ci.uy;
+    const R dlc = SED ? ci.c - cl : zero, drc = SED ? cr - ci.c : zero;
+    R deta, dux, duy, dc = zero;
+    if (lim == LIM_MINMOD) {
+        deta = lim_minmod(dle, dre);
+        dux = lim_minmod(dlu, dru);
+        duy = lim_minmod(dlv, drv);
+        if (SED) dc = lim_minmod(dlc, drc);
+    } else {
+        // Free-surface, velocity and concentration slopes all vanish in
+        // still water: one wave-wide vote skips their quotients when no lane
+        // takes one (the limited slopes are then the limiter's zeros).
+        const bool ue = vl_use(dle, dre), uu = vl_use(dlu, dru), uv = vl_use(dlv, drv);
+        const bool uc = SED && vl_use(dlc, drc);
+        deta = dux = duy = zero;
+        if (__builtin_amdgcn_ballot_w64(ue || uu || uv || uc) != 0) {
+            deta = vl_val(dle, dre, ue);
+            dux = vl_val(dlu, dru, uu);
+            duy = vl_val(dlv, drv, uv);
+            if (SED) dc = vl_val(dlc, drc, uc);
+        }
+    }
     R dz = dh - deta;
     Fm[3] = Hi - half * dz;
     Fp[3] = Hi + half * dz;
 
-    R du = limiter(lim, ci.ux - ulx, urx - ci.ux);
-    R dq = rmad(hi, du, ci.ux * dh);
+    R dq = rmad(hi, dux, ci.ux * dh);
     Fm[1] = rmad(-half, dq, ci.qx);
     Fp[1] = rmad(half, dq, ci.qx);
 
-    du = limiter(lim, ci.uy - uly, ury - ci.uy);
-    dq = rmad(hi, du, ci.uy * dh);
+    dq = rmad(hi, duy, ci.uy * dh);
     Fm[2] = rmad(-half, dq, ci.qy);
     Fp[2] = rmad(half, dq, ci.qy);
 
     if (SED) {
-        du = limiter(lim, ci.c - cl, cr - ci.c);
-        Fm[4] = rmad(-half, du, ci.c) * Fm[0];
-        Fp[4] = rmad(half, du, ci.c) * Fp[0];
+        Fm[4] = rmad(-half, dc, ci.c) * Fm[0];
+        Fp[4] = rmad(half, dc, ci.c) * Fp[0];
     } else {
         Fm[4] = Fp[4] = zero;
     }
@@ -484,14 +503,14 @@ __device__ __forceinline__ void stage_body(const StageArgs& a, const int tile)
             }
             Vel<R> un;
             // the quadratic law divides by h^n + DBL_EPSILON, the velocity's
-            // denominator: one shared reciprocal (FP64)
+            // denominator: one shared reciprocal
             const Den<R> Dh = den(v0[0] + (R)kEpsilon);
             if (a.k == 0) {   // U^n is this stage's input: its velocity is known
                 un.x = R0.ux;
                 un.y = R0.uy;
                 un.s = R0.us;
             } else if constexpr (COMP) {
-                un = velocity(v0[0], v0[1], v0[2]);
+                un = velocity(v0[0], v0[1], v0[2], &Dh.y);
             } else {
                 un = velocity64(v0[0], v0[1], v0[2], &Dh);
             }
