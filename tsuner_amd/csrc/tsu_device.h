@@ -345,9 +345,10 @@ __device__ __forceinline__ void cap_speed(Vel<double>& u)
 // one wave-wide test whether any lane of either side divides, then both
 // sides' reciprocals and quotients for every lane in one basic block, the
 // unused ones discarded by selects (h + DBL_EPSILON is a valid shared
-// denominator for any face depth).
+// denominator for any face depth).  Returns true (uniform over the wave)
+// when no lane divides: every velocity and concentration is then 0.
 template <bool SED>
-__device__ __forceinline__ void velocity_conc_pair(double h0, double qx0, double qy0, double hc0, double h1,
+__device__ __forceinline__ bool velocity_conc_pair(double h0, double qx0, double qy0, double hc0, double h1,
                                                    double qx1, double qy1, double hc1, Vel<double>& u0, double& c0,
                                                    Vel<double>& u1, double& c1)
 {
@@ -356,7 +357,7 @@ __device__ __forceinline__ void velocity_conc_pair(double h0, double qx0, double
     u0.x = u0.y = u0.s = 0.0;
     u1.x = u1.y = u1.s = 0.0;
     c0 = c1 = 0.0;
-    if (__builtin_amdgcn_ballot_w64(nu0 || nc0 || nu1 || nc1) == 0) return;
+    if (__builtin_amdgcn_ballot_w64(nu0 || nc0 || nu1 || nc1) == 0) return true;
     const Den<double> D0 = den(h0 + kEpsilon), D1 = den(h1 + kEpsilon);
     const double x0 = qdiv(qx0, D0), y0 = qdiv(qy0, D0);
     const double x1 = qdiv(qx1, D1), y1 = qdiv(qy1, D1);
@@ -372,6 +373,7 @@ __device__ __forceinline__ void velocity_conc_pair(double h0, double qx0, double
     u1.y = (nu1 && qy1 != 0.0) ? y1 : 0.0;
     cap_speed(u0);
     cap_speed(u1);
+    return false;
 }
 
 template <bool SED>
@@ -501,15 +503,28 @@ __device__ __forceinline__ void riemann(const DevParams& P, bool bed_mobile,
         for (int f = 0; f < (SED ? 5 : 3); f++) Dm[f] = Dp[f] = zero;
         return;
     }
+    // A lane at rest (equal hydrostatic depths, no velocity and no
+    // concentration on either side) has every flux, jump and bed term below
+    // equal to +-0 (the speeds and PVM coefficients stay finite; bedload is
+    // 0 at u = 0), so Dm = Dp = 0: when no lane of the wave moves, the solve
+    // is skipped.
     Vel<R> u0, u1;
     R c0, c1;
+    bool quiet;   // uniform: no lane has a velocity or a concentration
     if constexpr (sizeof(R) == 8) {
-        velocity_conc_pair<SED>(hl, qnl, qtl, hcl, hr, qnr, qtr, hcr, u0, c0, u1, c1);
+        quiet = velocity_conc_pair<SED>(hl, qnl, qtl, hcl, hr, qnr, qtr, hcr, u0, c0, u1, c1);
     } else {
         u0 = {qnl, qtl, qnl * qnl + qtl * qtl};
         u1 = {qnr, qtr, qnr * qnr + qtr * qtr};
         c0 = SED ? hcl : zero;
         c1 = SED ? hcr : zero;
+        bool moving = u0.x != zero || u0.y != zero || u1.x != zero || u1.y != zero;
+        if (SED) moving = moving || c0 != zero || c1 != zero;
+        quiet = __builtin_amdgcn_ballot_w64(moving) == 0;
+    }
+    if (quiet && __builtin_amdgcn_ballot_w64(h0 != h1) == 0) {
+        for (int f = 0; f < (SED ? 5 : 3); f++) Dm[f] = Dp[f] = zero;
+        return;
     }
     R q0n = u0.x * h0, q0t = u0.y * h0;
     R q1n = u1.x * h1, q1t = u1.y * h1;
