@@ -649,18 +649,43 @@ __device__ __forceinline__ R erosion_rate(const DevParams& P, R h, R modu, R hcv
     return Fe - Fd;
 }
 
-// Volume_kernel.cu:19-43
+// sqrt(x) for x >= 0: the unscaled square root unless a lane of the wave
+// has an argument below its range (FP64); the solver's square root (FP32).
+__device__ __forceinline__ double sqrt_voted(double x)
+{
+    return wave_any(sqrt_needs_scaling(x)) ? sqrt(x) : sqrt_unscaled0(x);
+}
+__device__ __forceinline__ float sqrt_voted(float x) { return rsqrt_(x); }
+
+// Volume_kernel.cu:19-43 in two parts: the wave speeds of one cell and the
+// dt of given speeds, cell_dt = dt_of_speeds(cell_speeds).  Division and
+// multiplication by a positive constant round monotonically, so the minimum
+// of cell_dt over a set of cells is dt_of_speeds of the set's largest speeds
+// (a dry cell has speeds 0 and dt = inf either way; speeds are >= 0): the
+// stage kernel reduces speeds and divides once per wave.
+template <typename R>
+__device__ __forceinline__ void cell_speeds(R h, R qx, R qy, R& lx, R& ly)
+{
+    Vel<R> u = velocity(h, qx, qy);
+    R cg = sqrt_voted((R)kG * h);
+    lx = rabs(u.x) + cg;
+    ly = rabs(u.y) + cg;
+}
+
+template <typename R>
+__device__ __forceinline__ R dt_of_speeds(const DevParams& P, R lx, R ly)
+{
+    // square cells: min(dx/lx, dx/ly) is dx/max(lx, ly) by the same monotonicity
+    if (P.dx == P.dy) return (R)P.cfl_half * rdiv((R)P.dx, rmax(lx, ly));
+    return (R)P.cfl_half * rmin(rdiv((R)P.dx, lx), rdiv((R)P.dy, ly));
+}
+
 template <typename R>
 __device__ __forceinline__ R cell_dt(const DevParams& P, R h, R qx, R qy)
 {
-    Vel<R> u = velocity(h, qx, qy);
-    R cg = rsqrt_((R)kG * h);
-    R lx = rabs(u.x) + cg;
-    R ly = rabs(u.y) + cg;
-    // square cells: rounding is monotone, so min(dx/lx, dx/ly) is dx/max(lx, ly)
-    // (lx, ly >= 0, never NaN; a dry cell gives dx/0 = inf either way)
-    if (P.dx == P.dy) return (R)P.cfl_half * rdiv((R)P.dx, rmax(lx, ly));
-    return (R)P.cfl_half * rmin(rdiv((R)P.dx, lx), rdiv((R)P.dy, ly));
+    R lx, ly;
+    cell_speeds(h, qx, qy, lx, ly);
+    return dt_of_speeds(P, lx, ly);
 }
 
 // Reduction_kernel_min.cu:155-196 applied to the exact minimum: for

@@ -77,6 +77,12 @@ __device__ __forceinline__ double wave_min(double v)
     for (int o = 32; o > 0; o >>= 1) v = fmin(v, __shfl_xor(v, o, 64));
     return v;
 }
+template <typename R>
+__device__ __forceinline__ R wave_max(R v)
+{
+    for (int o = 32; o > 0; o >>= 1) v = rmax(v, __shfl_xor(v, o, 64));
+    return v;
+}
 
 // Knuth's TwoSum: s + e == a + b exactly.
 __device__ __forceinline__ void two_sum(float a, float b, float& s, float& e)
@@ -379,7 +385,8 @@ __device__ __forceinline__ void stage_body(const StageArgs& a, const int tile)
         edge_flux<R, SED>(P, N_PY, fixed != 0, Um, D0, dm, FD0);
     }
 
-    double dtmin = __longlong_as_double((long long)kInfBits);
+    // largest wave speeds of the tile's updated cells (last stage: the next dt)
+    R lxmax = (R)0, lymax = (R)0;
 
     // ---- march over the rows of the tile -----------------------------------
     for (int r = y0; r < y1; ++r) {
@@ -517,7 +524,7 @@ __device__ __forceinline__ void stage_body(const StageArgs& a, const int tile)
             // |u| of U^n (friction, exchange); in FP64 a wave at rest skips
             // the square root of 0
             R modu = (R)0;
-            if (COMP || __builtin_amdgcn_ballot_w64(un.s != (R)0) != 0) modu = speed(un);
+            if (COMP || __builtin_amdgcn_ballot_w64(un.s != (R)0) != 0) modu = sqrt_voted(un.s);
             R cf;
             if (P.friction_law == FRIC_MANNING) {
                 R hp = v0[0] + (R)kEpsilon;
@@ -641,7 +648,10 @@ __device__ __forceinline__ void stage_body(const StageArgs& a, const int tile)
             if (a.last) {
                 // the next dt is evaluated on the stored state
                 const R h = W[0];
-                dtmin = fmin(dtmin, (double)cell_dt(P, h, W[1], W[2]));
+                R lx, ly;
+                cell_speeds(h, W[1], W[2], lx, ly);
+                lxmax = rmax(lxmax, lx);
+                lymax = rmax(lymax, ly);
                 if (a.ext) {
                     double hd = (double)h, Hs = (double)W[3];
                     double* hmax = a.ext;
@@ -668,7 +678,9 @@ __device__ __forceinline__ void stage_body(const StageArgs& a, const int tile)
     }
 
     if (a.last) {
-        double m = wave_min(dtmin);
+        lxmax = wave_max(lxmax);
+        lymax = wave_max(lymax);
+        const double m = (double)dt_of_speeds(P, lxmax, lymax);
         if (lane == 0 && m < __longlong_as_double((long long)kInfBits))
             atomicMin(&C->dt_bits[(step + 1) & 1], (unsigned long long)__double_as_longlong(m));
     }
