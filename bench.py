@@ -33,16 +33,14 @@ sys.path.insert(0, ROOT)
 HBM_PEAK_GBS = 8000.0
 # HBM bytes per stage-kernel launch on the 8192^2 workload, from rocprofv3
 # FETCH_SIZE and WRITE_SIZE runs scaled by what the same counters read for a
-# plain copy kernel of known size (x2.0 and x1.0); the summary with the
-# commands is profiles/stage_kernel_8192_counters.md.  None at other sizes.
-TRAFFIC_8192 = {"fp64": 7.595e9, "fp32": 6.262e9}
-# The stage kernels are bound by VALU issue, not HBM.  VALU instructions per
-# wave-row (one 64-lane wave over 60 cells of one row, one stage) on the
-# 8192^2 sediment workload, from rocprofv3 SQ_INSTS_VALU / (cells / 60) in the
-# same profile; the issue peak is MI355X's 1024 SIMDs x 2.4 GHz / 4 cycles per
-# wave64 VALU instruction (its published 78.6 TFLOP/s FP64 vector rate).
-VALU_PER_WAVE_ROW_8192 = {"fp64": 1357.0, "fp32": 1073.0}
-VALU_ISSUE_PEAK = 1024 * 2.4e9 / 4
+# plain copy kernel of known size; the summary with the commands, the build
+# (PROFILE_BUILD, a commit of this repository) and the scale factors is
+# TRAFFIC_PROFILE.  None at other sizes.  The stage kernels' work depends on
+# the state (waves at rest skip quotients and Riemann solves), their HBM
+# traffic does not.
+TRAFFIC_8192 = {"fp64": 7.404e9, "fp32": 6.099e9}
+TRAFFIC_PROFILE = "profiles/stage_kernel_8192_counters.md"
+PROFILE_BUILD = "8251093"
 METRIC = "cell-updates/s at 8192² FP64 vs FP32; ensemble sims/hr coarse+NN vs fine"
 
 
@@ -119,19 +117,13 @@ def run_variant(sc, precision, steps, warmup, tile_rows):
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS,
                      "traffic": TRAFFIC_8192[precision] if sc.npx == 8192 and sc.npy == 8192 else None,
+                     "traffic_source": ("%s (build %s)" % (TRAFFIC_PROFILE, PROFILE_BUILD)
+                                        if sc.npx == 8192 and sc.npy == 8192 else None),
                      "kernel": "stage_kernel<%s, %s>" % ("double, double" if precision == "fp64" else "float, float",
                                                          "true" if sed else "false"),
                      "algorithmic_bytes_per_launch": per_launch_bytes},
         "mass": g.mass(),
     }
-    if sc.npx == 8192 and sc.npy == 8192 and sed:
-        # wave-instructions per cell-update: two stages, 60 cells per wave-row
-        per_update = 2.0 * VALU_PER_WAVE_ROW_8192[precision] / 60.0
-        rate = res["value"] * per_update
-        res["valu_issue"] = {"bound": "valu", "achieved": rate, "peak": VALU_ISSUE_PEAK,
-                             "unit": "wave-instructions/s", "frac": rate / VALU_ISSUE_PEAK,
-                             "instructions_per_cell_update": per_update,
-                             "ceiling_cell_updates_per_s": VALU_ISSUE_PEAK / per_update}
     td = time.perf_counter()
     g.state()
     download_s = time.perf_counter() - td
@@ -506,13 +498,12 @@ def main():
                                % (args.size, args.size, sc.params["dx"], args.steps),
                    "cells": sc.npx * sc.npy, "bytes_per_cell_update": 200},
         "roofline": r64["roofline"],
-        "valu_issue": r64.get("valu_issue"),
         "cpu_baseline": cpu,
         "fp32": None if r32 is None else {"value": r32["value"], "ms_per_step": r32["ms_per_step"],
                                           "dtype": "f32 state and fluxes, compensated h/H/hc (+60 B/cell-update lo planes)",
                                           "bytes_per_cell_update": 100,
                                           "roofline": r32["roofline"],
-                                          "valu_issue": r32.get("valu_issue"),
+                                          "stage_ms_mean": r32["stage_ms_mean"],
                                           "pcie_inclusive": r32["pcie_inclusive"]},
         "stage_ms_mean": r64["stage_ms_mean"],
         "pcie_inclusive": r64["pcie_inclusive"],
