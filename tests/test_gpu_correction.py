@@ -65,7 +65,7 @@ def test_networks_meet_the_bars_and_beat_coarse_alone():
     the scenarios) at >= 8x the fine solver's throughput, each network
     improving on coarse alone in coastal peak, zone peak, nearshore mean and
     runup.  The zone peak (inundation front included) is reported, not held
-    to 5 cm: on 200 scenarios it is 7.5 cm (4x32) and 7.1 cm (64 wide)."""
+    to 5 cm: on 200 scenarios it is about 7.4 cm for both networks."""
     from tsuner_amd import correction as K
     nets = {"conv4_w32": K.load_net(K.WEIGHTS_4X32), "conv6_w64": K.load_net(K.WEIGHTS)}
     net, so, _ = nets["conv4_w32"]

@@ -95,6 +95,60 @@ def test_kat_edge_flux(oracle, law):
                                  % (i, L[i], R[i], normals[i], fixed[i], dm, dp, Dm[i], Dp[i]))
 
 
+def test_kat_edge_flux_waves_at_rest(oracle):
+    """The solve is skipped for a wave (64 consecutive edges) in which no
+    edge moves: equal hydrostatic depths and no velocity or concentration
+    on either side.  Blocks of 64 edges at rest with one lane that moves by
+    a depth jump, a tiny momentum, a tiny concentration or a step of the bed
+    under a level surface (which is still at rest), against the oracle."""
+    rng = np.random.default_rng(77)
+    blocks = []
+    for case in range(7):
+        h = rng.uniform(0.5, 20.0, 64)
+        H = rng.uniform(-5.0, 15.0, 64)
+        eta = h - H
+        L = np.stack([h, 0 * h, 0 * h, H, 0 * h], axis=1)
+        R = L.copy()
+        k = 17 + case
+        if case == 1:
+            R[k, 0] += 1e-3                     # depth jump
+        elif case == 2:
+            L[k, 1] = 1e-200                    # tiny normal momentum
+        elif case == 3:
+            R[k, 4] = 1e-300                    # tiny suspended load
+        elif case == 4:
+            R[:, 3] = H + rng.uniform(-2, 2, 64)   # bed steps under the level surface
+            R[:, 0] = eta + R[:, 3]
+            R[:, 0] = np.maximum(R[:, 0], 0.0)
+        elif case == 5:
+            L[::3, 0] = 0.0                     # dry-dry lanes among lanes at rest
+            R[::3, 0] = 0.0
+        elif case == 6:
+            L[k, 2] = -0.5                      # tangential momentum only
+        blocks.append((L, R))
+    L = np.ascontiguousarray(np.concatenate([b[0] for b in blocks]))
+    R = np.ascontiguousarray(np.concatenate([b[1] for b in blocks]))
+    n = len(L)
+    normals = np.tile(np.array([(1, 0), (0, 1), (-1, 0), (0, -1)], dtype=np.int32), (n // 4, 1))
+    fixed = np.ones(n, dtype=np.int32)
+    sc = S.sediment_shelf(n=8, dx=10.0, inflow=False)
+    p, s = sc.params, sc.scheme
+    Dm = np.zeros((n, 5))
+    Dp = np.zeros((n, 5))
+    lib = _lib.lib()
+    _lib.check(lib.tsu_kat_edge_flux(_lib.make_params(p), _lib.make_scheme(s), n, _lib.ptr(L), _lib.ptr(R),
+                                     _lib.ptr(np.ascontiguousarray(normals), _lib.C.c_int),
+                                     _lib.ptr(fixed, _lib.C.c_int), _lib.ptr(Dm), _lib.ptr(Dp), 0), "kat")
+    moving = 0
+    for i in range(n):
+        dm, dp = oracle.edge_flux(p, s, L[i], R[i], normals[i], fixed[i])
+        moving += bool(np.any(dm != 0) or np.any(dp != 0))
+        if not (np.array_equal(dm, Dm[i]) and np.array_equal(dp, Dp[i])):
+            raise AssertionError("edge %d (block %d): L=%r R=%r n=%r\n oracle Dm=%r Dp=%r\n gpu Dm=%r Dp=%r"
+                                 % (i, i // 64, L[i], R[i], normals[i], dm, dp, Dm[i], Dp[i]))
+    assert moving >= 4     # the single moving lanes do produce fluxes
+
+
 def kat_point(p, s, what, a, b=None, c=None, d=None):
     n = len(a)
     arr = [np.ascontiguousarray(x, dtype=np.float64) if x is not None else None for x in (a, b, c, d)]

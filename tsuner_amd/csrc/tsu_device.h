@@ -138,8 +138,10 @@ __device__ __forceinline__ bool vl_use(R a, R b)
         // |a + b| >= 2 sqrt(ab) > 0, so the reciprocal is finite.
         return a * b > 0.f;
     } else {
+        // same strict sign as ab > 0, except where ab underflows to 0: there
+        // the reference's numerator 2 RN(ab) is 0 and so is its quotient
         const R s = rabs(a) + rabs(b);
-        return ((a > (R)0 && b > (R)0) || (a < (R)0 && b < (R)0)) && s > (R)kEpsilon;
+        return a * b > (R)0 && s > (R)kEpsilon;
     }
 }
 
@@ -631,17 +633,22 @@ __device__ __forceinline__ void riemann(const DevParams& P, bool bed_mobile,
 
 // Reconstructor.cu:57-88 with SedFlux.cuh:51-99: exchange rate phi between
 // the bed (H) and the suspended load (hc), evaluated on U^n.
+// still (uniform over the wave) says |u| = 0 on every lane: then Z = 0,
+// Es = 0 / 1 = 0 and the erosion flux Fe is +0 without evaluating them.
 template <typename R>
-__device__ __forceinline__ R erosion_rate(const DevParams& P, R h, R modu, R hcv)
+__device__ __forceinline__ R erosion_rate(const DevParams& P, R h, R modu, R hcv, bool still = false)
 {
     R c = (R)0;
     if (hcv != (R)0) c = rdiv(hcv, h + (R)P.epsilon_h);
-    R Z = qdiv((R)P.ka * (modu * (R)P.sqrtFR), Den<R>{(R)P.vs, (R)P.inv_vs});
-    R Z5 = Z * Z * Z * Z * Z;
-    R Es = rdiv((R)1.3e-7 * Z5, (R)1 + (R)4.3e-7 * Z5);
-    R Fe = (R)P.vs1p * Es;
-    if (Fe > (R)0.1) Fe = (R)0.1;
-    Fe = (Fe > (R)0) ? Fe : (R)0;
+    R Fe = (R)0;
+    if (!still) {
+        R Z = qdiv((R)P.ka * (modu * (R)P.sqrtFR), Den<R>{(R)P.vs, (R)P.inv_vs});
+        R Z5 = Z * Z * Z * Z * Z;
+        R Es = rdiv((R)1.3e-7 * Z5, (R)1 + (R)4.3e-7 * Z5);
+        Fe = (R)P.vs1p * Es;
+        if (Fe > (R)0.1) Fe = (R)0.1;
+        Fe = (Fe > (R)0) ? Fe : (R)0;
+    }
     R Cb = (R)P.cbk * c;
     R Fd = (R)P.vs * Cb;
     if (Fd > (R)0.1) Fd = (R)0.1;

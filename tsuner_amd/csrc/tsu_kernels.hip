@@ -510,38 +510,55 @@ __device__ __forceinline__ void stage_body(const StageArgs& a, const int tile)
             }
             Vel<R> un;
             // the quadratic law divides by h^n + DBL_EPSILON, the velocity's
-            // denominator: one shared reciprocal
-            const Den<R> Dh = den(v0[0] + (R)kEpsilon);
+            // denominator: one shared reciprocal, computed where a quotient
+            // needs it (have_dh is uniform over the wave)
+            Den<R> Dh = {(R)1, (R)1};
+            bool have_dh = false;
+            if constexpr (COMP) {
+                Dh = den(v0[0] + (R)kEpsilon);
+                have_dh = true;
+            }
             if (a.k == 0) {   // U^n is this stage's input: its velocity is known
                 un.x = R0.ux;
                 un.y = R0.uy;
                 un.s = R0.us;
             } else if constexpr (COMP) {
                 un = velocity(v0[0], v0[1], v0[2], &Dh.y);
-            } else {
+            } else if (__builtin_amdgcn_ballot_w64(v0[0] >= (R)kEps && (v0[1] != (R)0 || v0[2] != (R)0)) != 0) {
+                Dh = den(v0[0] + (R)kEpsilon);
+                have_dh = true;
                 un = velocity64(v0[0], v0[1], v0[2], &Dh);
-            }
-            // |u| of U^n (friction, exchange); in FP64 a wave at rest skips
-            // the square root of 0
-            R modu = (R)0;
-            if (COMP || __builtin_amdgcn_ballot_w64(un.s != (R)0) != 0) modu = sqrt_voted(un.s);
-            R cf;
-            if (P.friction_law == FRIC_MANNING) {
-                R hp = v0[0] + (R)kEpsilon;
-                R dtGnn = (R)(dtd * kG * P.manning_n * P.manning_n);
-                if constexpr (COMP)
-                    cf = dtGnn * modu / (hp * cbrtf(hp));
-                else
-                    cf = dtGnn * modu / (hp * cbrt(hp));
             } else {
-                cf = qdiv(dtFR * modu, Dh);
+                un = velocity64(v0[0], v0[1], v0[2]);   // no lane divides
             }
-            // Reconstructor.cu:52-53: both momenta divided by 1 + c
-            const Den<R> Dc = den((R)1 + cf);
-            k1 = qdiv(k1, Dc);
-            k2 = qdiv(k2, Dc);
+            // |u| of U^n (friction, exchange).  In FP64 a wave at rest (|u| =
+            // 0 on every lane) has c = 0 and divides both momenta by 1 + 0 =
+            // 1, which leaves them unchanged: it skips the friction step (the
+            // FP32 solver, where the step is a few instructions, measured
+            // slower with the vote).
+            R modu = (R)0;
+            const bool still = !COMP && __builtin_amdgcn_ballot_w64(un.s != (R)0) == 0;
+            if (!still) {
+                modu = sqrt_voted(un.s);
+                if (!have_dh) Dh = den(v0[0] + (R)kEpsilon);
+                R cf;
+                if (P.friction_law == FRIC_MANNING) {
+                    R hp = v0[0] + (R)kEpsilon;
+                    R dtGnn = (R)(dtd * kG * P.manning_n * P.manning_n);
+                    if constexpr (COMP)
+                        cf = dtGnn * modu / (hp * cbrtf(hp));
+                    else
+                        cf = dtGnn * modu / (hp * cbrt(hp));
+                } else {
+                    cf = qdiv(dtFR * modu, Dh);
+                }
+                // Reconstructor.cu:52-53: both momenta divided by 1 + c
+                const Den<R> Dc = den((R)1 + cf);
+                k1 = qdiv(k1, Dc);
+                k2 = qdiv(k2, Dc);
+            }
             R phi = (R)0;
-            if (SED && P.suspension) phi = erosion_rate(P, v0[0], modu, v0[4]);
+            if (SED && P.suspension) phi = erosion_rate(P, v0[0], modu, v0[4], still);
 
             R W[5];
             W[1] = k1 - (invx * (FL[1] + FR[1]) + invy * (FD0[1] + FU0[1]));
