@@ -455,6 +455,57 @@ __device__ __forceinline__ R bedload(const DevParams& P, R h, const Vel<R>& u)
     }
 }
 
+// bedload for the two sides of an edge at once (values identical): with the
+// Ashida-Michiue law in FP64, one wave-wide test whether any lane of either
+// side moves its bed, then both sides' square roots and quotients for every
+// lane in one basic block, so that the two dependent chains interleave; the
+// unused results (lanes below a threshold, whose roots and quotients may be
+// NaN) are discarded by selects.  mobile gates both sides (the edge's bed
+// mask).
+template <typename R>
+__device__ __forceinline__ void bedload_pair(const DevParams& P, bool mobile, R h0, const Vel<R>& u0, R h1,
+                                             const Vel<R>& u1, R& f0, R& f1)
+{
+    if constexpr (sizeof(R) == 8) {
+        if (P.bedload_law == BEDLOAD_AM) {
+            const double thr = P.OC * (1.0 - 0x1p-40);
+            const bool a0 = mobile && h0 > P.HQB && h0 > P.fiveDS && P.Opar * u0.s > thr;
+            const bool a1 = mobile && h1 > P.HQB && h1 > P.fiveDS && P.Opar * u1.s > thr;
+            f0 = f1 = 0.0;
+            if (__builtin_amdgcn_ballot_w64(a0 || a1) == 0) return;
+            double m0, m1;
+            if (P.bed_sqrt_unscaled) {
+                m0 = sqrt_unscaled(u0.s);
+                m1 = sqrt_unscaled(u1.s);
+            } else {
+                m0 = speed(u0);
+                m1 = speed(u1);
+            }
+            const double O0 = P.Opar * m0 * m0, O1 = P.Opar * m1 * m1;
+            const bool b0 = a0 && O0 > P.OC, b1 = a1 && O1 > P.OC;
+            const double K0 = (17.0 * u0.x) / (m0 + kEpsilon), K1 = (17.0 * u1.x) / (m1 + kEpsilon);
+            double s0, s1;
+            if (P.bed_sqrt_unscaled) {
+                s0 = sqrt_unscaled(O0);
+                s1 = sqrt_unscaled(O1);
+            } else {
+                s0 = sqrt(O0);
+                s1 = sqrt(O1);
+            }
+            const double q0 = K0 * P.Q * (O0 - P.OC) * fmax(s0 - P.sqrtOC, 0.0);
+            const double q1 = K1 * P.Q * (O1 - P.OC) * fmax(s1 - P.sqrtOC, 0.0);
+            f0 = b0 ? -q0 : 0.0;
+            f1 = b1 ? -q1 : 0.0;
+            return;
+        }
+    }
+    f0 = f1 = (R)0;
+    if (mobile) {
+        f0 = bedload(P, h0, u0);
+        f1 = bedload(P, h1, u1);
+    }
+}
+
 // Unrotated (qx, qy) -> normal frame (qn, qt) for the four axis normals
 // (VolumeSolver.cu:28-31 with normal_x, normal_y in {-1, 0, 1}).
 template <typename R>
@@ -607,11 +658,8 @@ __device__ __forceinline__ void riemann(const DevParams& P, bool bed_mobile,
 
     if (SED) {
         R hc0 = h0 * c0, hc1 = h1 * c1;
-        R F03 = zero, F13 = zero;
-        if (bed_mobile) {
-            F03 = bedload(P, h0, u0);
-            F13 = bedload(P, h1, u1);
-        }
+        R F03, F13;
+        bedload_pair(P, bed_mobile, h0, u0, h1, u1, F03, F13);
         R F04 = hc0 * u0.x, F14 = hc1 * u1.x;
         R df3 = F13 - F03, df4 = F14 - F04;
         R DU4 = hc1 - hc0;
