@@ -68,9 +68,14 @@ def random_faces(rng, n):
     return np.stack([h, qx, qy, H, hc], axis=1)
 
 
-@pytest.mark.parametrize("law", [S.BEDLOAD_AM, S.BEDLOAD_GRASS])
-def test_kat_edge_flux(oracle, law):
-    rng = np.random.default_rng(11 + law)
+@pytest.mark.parametrize("law, tiny_shields", [(S.BEDLOAD_AM, False), (S.BEDLOAD_GRASS, False),
+                                               (S.BEDLOAD_AM, True)])
+def test_kat_edge_flux(oracle, law, tiny_shields):
+    """Edge fluxes on random wet/dry faces, bitwise against the oracle.  The
+    Ashida-Michiue law takes the stage kernels' branch-free bedload pair for
+    the configured constants; with a critical Shields number of 1e-220 the
+    square roots may need operand scaling and the general bedload path runs."""
+    rng = np.random.default_rng(11 + law + 5 * tiny_shields)
     n = 3000
     L, R = random_faces(rng, n), random_faces(rng, n)
     # shallow subset so bedload thresholds (h > 5 DS) and theta > theta_c are crossed
@@ -80,6 +85,8 @@ def test_kat_edge_flux(oracle, law):
     fixed = rng.integers(0, 2, n).astype(np.int32)
     sc = S.sediment_shelf(n=8, dx=10.0, inflow=False, bedload_law=law)
     p, s = sc.params, sc.scheme
+    if tiny_shields:
+        p = dict(p, OC=1e-220)
     Dm = np.zeros((n, 5))
     Dp = np.zeros((n, 5))
     L_ = np.ascontiguousarray(L)

@@ -297,7 +297,7 @@ int enqueue_step(tsu_state* st)
             st->ev_stage[st->ev_used] = k;
             HIPCHK(hipEventRecord(st->ev[2 * st->ev_used], st->stream));
         }
-        HIPCHK(tsu::launch_stage(a, st->sed, st->precision, st->stream));
+        HIPCHK(tsu::launch_stage(a, st->sed, st->precision, st->stream, tsu::fast_bedload(st->P)));
         if (timed) {
             HIPCHK(hipEventRecord(st->ev[2 * st->ev_used + 1], st->stream));
             st->ev_used++;

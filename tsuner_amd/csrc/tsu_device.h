@@ -506,29 +506,58 @@ __device__ __forceinline__ void bedload_pair(const DevParams& P, bool mobile, R 
     }
 }
 
+// Run constants for which the stage kernels take bedload_am_pair: the
+// Ashida-Michiue law with both bedload square roots in the unscaled range.
+__host__ __device__ inline bool fast_bedload(const DevParams& P)
+{
+    return P.bedload && P.bedload_law == BEDLOAD_AM && P.bed_sqrt_unscaled;
+}
+
+// bedload_pair in FP64 for fast_bedload run constants, without the vote
+// and without branches: every lane evaluates both chains and the selects keep
+// the lanes past the thresholds (values identical).  Placed among the HLL
+// arithmetic of the Riemann solver, it shares that basic block, so the
+// scheduler interleaves its square root -> quotient -> square root chains
+// with the HLL chains.
+__device__ __forceinline__ void bedload_am_pair(const DevParams& P, bool mobile, double h0, const Vel<double>& u0,
+                                                double h1, const Vel<double>& u1, double& f0, double& f1)
+{
+    const double thr = P.OC * (1.0 - 0x1p-40);
+    const bool a0 = mobile && h0 > P.HQB && h0 > P.fiveDS && P.Opar * u0.s > thr;
+    const bool a1 = mobile && h1 > P.HQB && h1 > P.fiveDS && P.Opar * u1.s > thr;
+    const double m0 = sqrt_unscaled(u0.s), m1 = sqrt_unscaled(u1.s);
+    const double O0 = P.Opar * m0 * m0, O1 = P.Opar * m1 * m1;
+    const bool b0 = a0 && O0 > P.OC, b1 = a1 && O1 > P.OC;
+    const double K0 = (17.0 * u0.x) / (m0 + kEpsilon), K1 = (17.0 * u1.x) / (m1 + kEpsilon);
+    const double s0 = sqrt_unscaled(O0), s1 = sqrt_unscaled(O1);
+    const double q0 = K0 * P.Q * (O0 - P.OC) * fmax(s0 - P.sqrtOC, 0.0);
+    const double q1 = K1 * P.Q * (O1 - P.OC) * fmax(s1 - P.sqrtOC, 0.0);
+    f0 = b0 ? -q0 : 0.0;
+    f1 = b1 ? -q1 : 0.0;
+}
+
 // Unrotated (qx, qy) -> normal frame (qn, qt) for the four axis normals
-// (VolumeSolver.cu:28-31 with normal_x, normal_y in {-1, 0, 1}).
+// (VolumeSolver.cu:28-31 with normal_x, normal_y in {-1, 0, 1}): +x (qx, qy),
+// -x (-qx, -qy), +y (qy, -qx), -y (-qy, qx).  Written as selects and a
+// negation (the kind may differ between lanes), no branches.
 template <typename R>
 __device__ __forceinline__ void to_normal(int kind, R qx, R qy, R& qn, R& qt)
 {
-    switch (kind) {
-    case N_PX: qn = qx; qt = qy; break;
-    case N_MX: qn = -qx; qt = -qy; break;
-    case N_PY: qn = qy; qt = -qx; break;
-    default:   qn = -qy; qt = qx; break;
-    }
+    const bool ydir = kind == N_PY || kind == N_MY, neg = kind == N_MX || kind == N_MY;
+    const R n = ydir ? qy : qx, t = ydir ? -qx : qy;
+    qn = neg ? -n : n;
+    qt = neg ? -t : t;
 }
 
-// Normal frame -> unrotated (VolumeSolver.cu:142-152).
+// Normal frame -> unrotated (VolumeSolver.cu:142-152): +x (a, b), -x (-a, -b),
+// +y (-b, a), -y (b, -a).
 template <typename R>
 __device__ __forceinline__ void from_normal(int kind, R a, R b, R& fx, R& fy)
 {
-    switch (kind) {
-    case N_PX: fx = a; fy = b; break;
-    case N_MX: fx = -a; fy = -b; break;
-    case N_PY: fx = -b; fy = a; break;
-    default:   fx = b; fy = -a; break;
-    }
+    const bool ydir = kind == N_PY || kind == N_MY, neg = kind == N_MX || kind == N_MY;
+    const R x = ydir ? -b : a, y = ydir ? a : b;
+    fx = neg ? -x : x;
+    fy = neg ? -y : y;
 }
 
 // VolumeSolver.cu:19-126 in the normal frame: hydrostatic reconstruction and
@@ -539,7 +568,9 @@ __device__ __forceinline__ void from_normal(int kind, R a, R b, R& fx, R& fy)
 // order) already computed, which the FP64 path recomputes from h u and h c.
 // With SED false the bed and concentration rows are identically zero (no
 // bedload, no suspension, hc == 0) and are not computed.
-template <typename R, bool SED>
+// FB (FP64 with sediment, fast_bedload(P) true): bedload by bedload_am_pair
+// among the HLL arithmetic; otherwise by bedload_pair with the bed terms.
+template <typename R, bool SED, bool FB = false>
 __device__ __forceinline__ void riemann(const DevParams& P, bool bed_mobile,
                                         R hl, R zl, R qnl, R qtl, R hcl,
                                         R hr, R zr, R qnr, R qtr, R hcr,
@@ -616,6 +647,8 @@ __device__ __forceinline__ void riemann(const DevParams& P, bool bed_mobile,
         SR = u1.x + sg * sh1;
         cm = sg * rsqrt_(hm);
     }
+    R F03 = zero, F13 = zero;
+    if constexpr (FB && SED && sizeof(R) == 8) bedload_am_pair(P, bed_mobile, h0, u0, h1, u1, F03, F13);
     // h0 + h1 > 0 here, so sqrt(h0) + sqrt(h1) > 0: the Roe velocity is
     // always defined (the reference's 0/0 of two dry sides exits above).
     R ssum = sh0 + sh1;
@@ -658,8 +691,7 @@ __device__ __forceinline__ void riemann(const DevParams& P, bool bed_mobile,
 
     if (SED) {
         R hc0 = h0 * c0, hc1 = h1 * c1;
-        R F03, F13;
-        bedload_pair(P, bed_mobile, h0, u0, h1, u1, F03, F13);
+        if constexpr (!(FB && sizeof(R) == 8)) bedload_pair(P, bed_mobile, h0, u0, h1, u1, F03, F13);
         R F04 = hc0 * u0.x, F14 = hc1 * u1.x;
         R df3 = F13 - F03, df4 = F14 - F04;
         R DU4 = hc1 - hc0;

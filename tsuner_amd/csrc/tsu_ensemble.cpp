@@ -185,9 +185,11 @@ int enqueue_step(tsu_ensemble* e)
 {
     const int order = e->s[0].order;
     const tsu::BatchPools pools{e->A, e->B, e->fix, e->ctrl, e->ext, e->dP};
+    bool fast_bed = true;
+    for (int i = 0; i < e->n; i++) fast_bed = fast_bed && tsu::fast_bedload(e->P[i]);
     for (int k = 0; k < order; k++)
         ECHK(tsu::launch_stage_batch(pools, e->dmem, e->tiles, e->ntiles, k, k == order - 1, e->sed, e->precision,
-                                     e->stream));
+                                     e->stream, fast_bed));
     ECHK(tsu::launch_advance_batch(e->ctrl, e->dP, e->n, e->stream));
     return TSU_OK;
 }

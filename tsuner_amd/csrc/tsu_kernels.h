@@ -29,8 +29,10 @@ struct StageArgs {
     const DevParams* P;   // run constants in device memory
 };
 
-// precision: 0 = FP64 planes, 1 = FP32 planes (FP64 arithmetic).
-hipError_t launch_stage(const StageArgs& a, bool sed, int precision, hipStream_t s);
+// precision: 0 = FP64 planes, 1 = FP32 planes (FP64 arithmetic).  fast_bed:
+// the run constants satisfy fast_bedload (tsu_device.h); FP64 with sediment
+// then takes the kernel that evaluates bedload among the HLL arithmetic.
+hipError_t launch_stage(const StageArgs& a, bool sed, int precision, hipStream_t s, bool fast_bed = false);
 
 // Ensemble member as seen by the batched stage kernel: byte offsets into
 // the ensemble's pools instead of pointers, so that every address the kernel
@@ -52,8 +54,9 @@ struct BatchPools {
     const DevParams* P;   // one per member
 };
 // Ensemble: one launch over all tiles of all members (tiles[i] = member, tile).
+// fast_bed: every member's run constants satisfy fast_bedload.
 hipError_t launch_stage_batch(const BatchPools& pools, const BatchMember* mem, const int2* tiles, int ntiles, int k,
-                              int last, bool sed, int precision, hipStream_t s);
+                              int last, bool sed, int precision, hipStream_t s, bool fast_bed = false);
 hipError_t launch_advance_batch(Ctrl* ctrl, const DevParams* P, int n, hipStream_t s);
 hipError_t launch_init_dt(const void* V, const DevParams& P, Ctrl* ctrl, int precision, hipStream_t s);
 hipError_t launch_warmup(hipStream_t s);

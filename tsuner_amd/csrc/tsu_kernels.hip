@@ -275,14 +275,14 @@ __device__ __forceinline__ void cell_face(const Cell<R>& c, R* F)
 
 // Riemann problem on one edge with left/right faces in the unrotated frame;
 // Dm, Dp come back in the unrotated frame (VolumeSolver.cu:19-170).
-template <typename R, bool SED>
+template <typename R, bool SED, bool FB = false>
 __device__ __forceinline__ void edge_flux(const DevParams& P, int kind, bool mobile, const R* Lf, const R* Rf, R* Dm,
                                           R* Dp)
 {
     R qnl, qtl, qnr, qtr, dm[5], dp[5];
     to_normal(kind, Lf[1], Lf[2], qnl, qtl);
     to_normal(kind, Rf[1], Rf[2], qnr, qtr);
-    riemann<R, SED>(P, mobile, Lf[0], Lf[3], qnl, qtl, Lf[4], Rf[0], Rf[3], qnr, qtr, Rf[4], dm, dp);
+    riemann<R, SED, FB>(P, mobile, Lf[0], Lf[3], qnl, qtl, Lf[4], Rf[0], Rf[3], qnr, qtr, Rf[4], dm, dp);
     Dm[0] = dm[0];
     Dp[0] = dp[0];
     from_normal(kind, dm[1], dm[2], Dm[1], Dm[2]);
@@ -297,7 +297,8 @@ __device__ __forceinline__ void edge_flux(const DevParams& P, int kind, bool mob
     }
 }
 
-template <typename R, typename S, bool SED>
+// FB: the run constants satisfy fast_bedload (FP64 with sediment only).
+template <typename R, typename S, bool SED, bool FB>
 __device__ __forceinline__ void stage_body(const StageArgs& a, const int tile)
 {
     constexpr bool COMP = sizeof(R) == 4;   // FP32 solver: compensated h, H, hc
@@ -367,7 +368,7 @@ __device__ __forceinline__ void stage_body(const StageArgs& a, const int tile)
     if (y0 == 0 && !pery) {
         R G5[5], dp[5];
         ghost_state(P, 2, D0, G5, (long long)xc, 2, ct);
-        edge_flux<R, SED>(P, N_MY, fx0 != 0, D0, G5, FD0, dp);
+        edge_flux<R, SED, FB>(P, N_MY, fx0 != 0, D0, G5, FD0, dp);
     } else {
         R Um[5], Dd[5], dm[5];
         if (second) {
@@ -382,7 +383,7 @@ __device__ __forceinline__ void stage_body(const StageArgs& a, const int tile)
         }
         int fxm = fix_at(y0 - 1);
         int fixed = (y0 == 0) ? fxm : fxm * fx0;   // periodic wrap edge: VolumeSolver.cu:386
-        edge_flux<R, SED>(P, N_PY, fixed != 0, Um, D0, dm, FD0);
+        edge_flux<R, SED, FB>(P, N_PY, fixed != 0, Um, D0, dm, FD0);
     }
 
     // largest wave speeds of the tile's updated cells (last stage: the next dt)
@@ -462,7 +463,7 @@ __device__ __forceinline__ void stage_body(const StageArgs& a, const int tile)
             fixed = (perx && (x == 0 || x == npx)) ? fxl : fxl * fx0;
         }
         R Dmx[5], Dpx[5];
-        edge_flux<R, SED>(P, kind, fixed != 0, El, Er, Dmx, Dpx);
+        edge_flux<R, SED, FB>(P, kind, fixed != 0, El, Er, Dmx, Dpx);
         R FL[5], FR[5];
         const bool left_wall = (x == 0 && !perx);
         for (int f = 0; f < 5; f++) {
@@ -485,10 +486,10 @@ __device__ __forceinline__ void stage_body(const StageArgs& a, const int tile)
         if (r + 1 == npy && !pery) {
             R G5[5], dp[5];
             ghost_state(P, 3, U0, G5, (long long)npx + npy + xc, 2, ct);
-            edge_flux<R, SED>(P, N_PY, fx0 != 0, U0, G5, FU0, dp);
+            edge_flux<R, SED, FB>(P, N_PY, fx0 != 0, U0, G5, FU0, dp);
         } else {
             int fy = (r + 1 == npy) ? fx0 : fx0 * fx1;
-            edge_flux<R, SED>(P, N_PY, fy != 0, U0, D1, FU0, FD1);
+            edge_flux<R, SED, FB>(P, N_PY, fy != 0, U0, D1, FU0, FD1);
         }
 
         // cell update of row r (Reconstructor.cu:197,274,353-354;
@@ -723,17 +724,17 @@ constexpr int stage_waves()
     return sizeof(R) == 4 ? TSU_WAVES_FP32 : (SED ? TSU_WAVES_FP64 : TSU_WAVES_FP64_NOSED);
 }
 
-template <typename R, typename S, bool SED>
+template <typename R, typename S, bool SED, bool FB = false>
 __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(stage_waves<R, SED>(), stage_waves<R, SED>())))
 void stage_kernel(StageArgs a)
 {
-    stage_body<R, S, SED>(a, xcd_tile(blockIdx.x, gridDim.x));
+    stage_body<R, S, SED, FB>(a, xcd_tile(blockIdx.x, gridDim.x));
 }
 
 // Ensemble: one launch covers every tile of every scenario.  tiles[b] =
 // (scenario, tile within the scenario); scen[s] holds that scenario's
 // buffers, constants and step-control block.
-template <typename R, typename S, bool SED>
+template <typename R, typename S, bool SED, bool FB = false>
 __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(stage_waves<R, SED>(), stage_waves<R, SED>())))
 void stage_kernel_batch(BatchPools pools, const BatchMember* __restrict__ mem, const int2* __restrict__ tiles, int k,
                         int last)
@@ -766,7 +767,7 @@ void stage_kernel_batch(BatchPools pools, const BatchMember* __restrict__ mem, c
     a.nstrips = m.nstrips;
     a.ntiles = m.ntiles;
     a.P = pools.P + e.x;
-    stage_body<R, S, SED>(a, e.y);
+    stage_body<R, S, SED, FB>(a, e.y);
 }
 
 // GPU/main_TSUNER.cu:147-148: dt of the initial state into dt_bits[step & 1].
@@ -884,6 +885,7 @@ __global__ void mass_final_kernel(const double* partial, int nb, double* out)
     out[1] = b;
 }
 
+template <bool FB>
 __global__ void kat_edge_kernel(DevParams P, int n, const double* left, const double* right, const int* kind,
                                 const int* fixed, double* Dm, double* Dp)
 {
@@ -894,7 +896,7 @@ __global__ void kat_edge_kernel(DevParams P, int n, const double* left, const do
         L[f] = left[5 * i + f];
         R[f] = right[5 * i + f];
     }
-    edge_flux<double, true>(P, kind[i], fixed[i] != 0 && P.bedload, L, R, dm, dp);
+    edge_flux<double, true, FB>(P, kind[i], fixed[i] != 0 && P.bedload, L, R, dm, dp);
     for (int f = 0; f < 5; f++) {
         Dm[5 * i + f] = dm[f];
         Dp[5 * i + f] = dp[f];
@@ -965,11 +967,13 @@ __global__ void kat_point_kernel(int what, int law, int n, const double* a, cons
 
 }  // namespace
 
-hipError_t launch_stage(const StageArgs& a, bool sed, int precision, hipStream_t s)
+hipError_t launch_stage(const StageArgs& a, bool sed, int precision, hipStream_t s, bool fast_bed)
 {
     dim3 grid(a.ntiles), block(64);
     if (precision == 0) {
-        if (sed)
+        if (sed && fast_bed)
+            hipLaunchKernelGGL((stage_kernel<double, double, true, true>), grid, block, 0, s, a);
+        else if (sed)
             hipLaunchKernelGGL((stage_kernel<double, double, true>), grid, block, 0, s, a);
         else
             hipLaunchKernelGGL((stage_kernel<double, double, false>), grid, block, 0, s, a);
@@ -983,11 +987,14 @@ hipError_t launch_stage(const StageArgs& a, bool sed, int precision, hipStream_t
 }
 
 hipError_t launch_stage_batch(const BatchPools& pools, const BatchMember* mem, const int2* tiles, int ntiles, int k,
-                              int last, bool sed, int precision, hipStream_t s)
+                              int last, bool sed, int precision, hipStream_t s, bool fast_bed)
 {
     dim3 grid(ntiles), block(64);
     if (precision == 0) {
-        if (sed)
+        if (sed && fast_bed)
+            hipLaunchKernelGGL((stage_kernel_batch<double, double, true, true>), grid, block, 0, s, pools, mem, tiles, k,
+                               last);
+        else if (sed)
             hipLaunchKernelGGL((stage_kernel_batch<double, double, true>), grid, block, 0, s, pools, mem, tiles, k, last);
         else
             hipLaunchKernelGGL((stage_kernel_batch<double, double, false>), grid, block, 0, s, pools, mem, tiles, k,
@@ -1071,7 +1078,13 @@ hipError_t launch_mass(const void* V, long long nvol, int precision, double* par
 hipError_t launch_kat_edge(const DevParams& P, int n, const double* left, const double* right, const int* kind,
                            const int* fixed, double* Dm, double* Dp, hipStream_t s)
 {
-    hipLaunchKernelGGL(kat_edge_kernel, dim3((n + 63) / 64), dim3(64), 0, s, P, n, left, right, kind, fixed, Dm, Dp);
+    // the stage kernels' choice of bedload evaluation for these run constants
+    if (fast_bedload(P))
+        hipLaunchKernelGGL(kat_edge_kernel<true>, dim3((n + 63) / 64), dim3(64), 0, s, P, n, left, right, kind, fixed,
+                           Dm, Dp);
+    else
+        hipLaunchKernelGGL(kat_edge_kernel<false>, dim3((n + 63) / 64), dim3(64), 0, s, P, n, left, right, kind,
+                           fixed, Dm, Dp);
     return hipGetLastError();
 }
 
