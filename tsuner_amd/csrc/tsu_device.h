@@ -536,6 +536,24 @@ __device__ __forceinline__ void bedload_am_pair(const DevParams& P, bool mobile,
     f1 = b1 ? -q1 : 0.0;
 }
 
+// The FP32 solver's bedload pair in the same branch-free form (its square
+// root and reciprocal need no range condition).
+__device__ __forceinline__ void bedload_am_pair(const DevParams& P, bool mobile, float h0, const Vel<float>& u0,
+                                                float h1, const Vel<float>& u1, float& f0, float& f1)
+{
+    const float Opar = (float)P.Opar, OC = (float)P.OC;
+    const float O0 = Opar * u0.s, O1 = Opar * u1.s;
+    const bool b0 = mobile && h0 > (float)P.HQB && h0 > (float)P.fiveDS && O0 > OC;
+    const bool b1 = mobile && h1 > (float)P.HQB && h1 > (float)P.fiveDS && O1 > OC;
+    const float K0 = 17.0f * u0.x * __builtin_amdgcn_rcpf(rsqrt_(u0.s) + (float)kEpsilon);
+    const float K1 = 17.0f * u1.x * __builtin_amdgcn_rcpf(rsqrt_(u1.s) + (float)kEpsilon);
+    const float sq = (float)P.sqrtOC, Q = (float)P.Q;
+    const float q0 = K0 * Q * (O0 - OC) * fmaxf(rsqrt_(O0) - sq, 0.f);
+    const float q1 = K1 * Q * (O1 - OC) * fmaxf(rsqrt_(O1) - sq, 0.f);
+    f0 = b0 ? -q0 : 0.f;
+    f1 = b1 ? -q1 : 0.f;
+}
+
 // Unrotated (qx, qy) -> normal frame (qn, qt) for the four axis normals
 // (VolumeSolver.cu:28-31 with normal_x, normal_y in {-1, 0, 1}): +x (qx, qy),
 // -x (-qx, -qy), +y (qy, -qx), -y (-qy, qx).  Written as selects and a
@@ -648,7 +666,7 @@ __device__ __forceinline__ void riemann(const DevParams& P, bool bed_mobile,
         cm = sg * rsqrt_(hm);
     }
     R F03 = zero, F13 = zero;
-    if constexpr (FB && SED && sizeof(R) == 8) bedload_am_pair(P, bed_mobile, h0, u0, h1, u1, F03, F13);
+    if constexpr (FB && SED) bedload_am_pair(P, bed_mobile, h0, u0, h1, u1, F03, F13);
     // h0 + h1 > 0 here, so sqrt(h0) + sqrt(h1) > 0: the Roe velocity is
     // always defined (the reference's 0/0 of two dry sides exits above).
     R ssum = sh0 + sh1;
@@ -691,7 +709,7 @@ __device__ __forceinline__ void riemann(const DevParams& P, bool bed_mobile,
 
     if (SED) {
         R hc0 = h0 * c0, hc1 = h1 * c1;
-        if constexpr (!(FB && sizeof(R) == 8)) bedload_pair(P, bed_mobile, h0, u0, h1, u1, F03, F13);
+        if constexpr (!FB) bedload_pair(P, bed_mobile, h0, u0, h1, u1, F03, F13);
         R F04 = hc0 * u0.x, F14 = hc1 * u1.x;
         R df3 = F13 - F03, df4 = F14 - F04;
         R DU4 = hc1 - hc0;

@@ -978,7 +978,9 @@ hipError_t launch_stage(const StageArgs& a, bool sed, int precision, hipStream_t
         else
             hipLaunchKernelGGL((stage_kernel<double, double, false>), grid, block, 0, s, a);
     } else {
-        if (sed)
+        if (sed && fast_bed)
+            hipLaunchKernelGGL((stage_kernel<float, float, true, true>), grid, block, 0, s, a);
+        else if (sed)
             hipLaunchKernelGGL((stage_kernel<float, float, true>), grid, block, 0, s, a);
         else
             hipLaunchKernelGGL((stage_kernel<float, float, false>), grid, block, 0, s, a);
@@ -1000,7 +1002,10 @@ hipError_t launch_stage_batch(const BatchPools& pools, const BatchMember* mem, c
             hipLaunchKernelGGL((stage_kernel_batch<double, double, false>), grid, block, 0, s, pools, mem, tiles, k,
                                last);
     } else {
-        if (sed)
+        if (sed && fast_bed)
+            hipLaunchKernelGGL((stage_kernel_batch<float, float, true, true>), grid, block, 0, s, pools, mem, tiles, k,
+                               last);
+        else if (sed)
             hipLaunchKernelGGL((stage_kernel_batch<float, float, true>), grid, block, 0, s, pools, mem, tiles, k, last);
         else
             hipLaunchKernelGGL((stage_kernel_batch<float, float, false>), grid, block, 0, s, pools, mem, tiles, k,
