@@ -79,13 +79,13 @@ def cpu_model():
 
 
 def stage_kernel_name(sc, precision):
-    """The stage kernel the library launches for this workload: FP64 with
-    sediment takes the variant that evaluates bedload among the HLL
-    arithmetic when the run uses the Ashida-Michiue law (the benchmark's
-    constants keep its square roots in the unscaled range)."""
+    """The stage kernel the library launches for this workload: with
+    sediment, the variant that evaluates bedload among the HLL arithmetic
+    when the run uses the Ashida-Michiue law (the benchmark's constants keep
+    its FP64 square roots in the unscaled range)."""
     from tsuner_amd import scenarios as S
     sed = bool(sc.scheme["bedload"] or sc.scheme["suspension"])
-    fast = precision == "fp64" and sed and bool(sc.scheme["bedload"]) and sc.scheme["bedload_law"] == S.BEDLOAD_AM
+    fast = sed and bool(sc.scheme["bedload"]) and sc.scheme["bedload_law"] == S.BEDLOAD_AM
     return "stage_kernel<%s, %s, %s>" % ("double, double" if precision == "fp64" else "float, float",
                                          "true" if sed else "false", "true" if fast else "false")
 
