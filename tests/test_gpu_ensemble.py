@@ -31,6 +31,25 @@ def test_fp64_members_equal_oracle_and_single(oracle):
         assert (t, dt, n) == (o.t, o.dt, 12)
 
 
+def test_fp64_mixed_bedload_laws_equal_oracle(oracle):
+    """A member with Grass bedload makes the whole ensemble take the general
+    bedload kernel; the Ashida-Michiue members must still equal the oracle
+    bit for bit (their single-grid runs take the branch-free variant)."""
+    scs = [S.gaussian_source(n=48, dx=40.0, seed=1),
+           S.gaussian_source(n=70, dx=40.0, seed=2, bedload_law=S.BEDLOAD_GRASS, grass_ag=1e-3),
+           S.gaussian_source(n=61, dx=40.0, seed=3)]
+    ens = Ensemble(scs, precision="fp64", tile_rows=[8, 16, 5])
+    ens.step(12)
+    for i, sc in enumerate(scs):
+        print("member %d %s law %d" % (i, sc.name, sc.scheme["bedload_law"]), flush=True)
+        o = oracle.from_scenario(sc)
+        o.step(12)
+        assert np.array_equal(ens.state(i), o.state()), "member %d differs from the oracle" % i
+        g = Solver.from_scenario(sc, precision="fp64", tile_rows=16)
+        g.step(12)
+        assert np.array_equal(ens.state(i), g.state()), "member %d differs from its single-grid run" % i
+
+
 def test_fp32_members_equal_single_solver():
     scs = members()
     ens = Ensemble(scs, precision="fp32")
