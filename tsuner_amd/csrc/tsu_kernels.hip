@@ -705,10 +705,10 @@ __device__ __forceinline__ void stage_body(const StageArgs& a, const int tile)
 }
 
 // Waves per SIMD the register allocation targets.  Measured on the 8192^2
-// bench: FP32 at 3 waves (167 VGPRs, a few spilled) runs 24% faster than at
-// 2 (173 VGPRs); 4 waves spill 39 VGPRs and lose.  FP64 with sediment needs
-// 226 VGPRs; at 3 waves it spills ~90 and runs at half speed, so it stays at
-// 2.  FP64 without sediment fits 168 VGPRs, i.e. 3 waves, unspilled.
+// bench: FP32 at 3 waves (168 VGPRs, a few spilled) runs 24% faster than at
+// 2; 4 waves spill 39 VGPRs and lose.  FP64 with sediment needs 245-255
+// VGPRs; at 3 waves it spills 79 or more and runs at half speed, so it stays
+// at 2.  FP64 without sediment fits 168 VGPRs, i.e. 3 waves, unspilled.
 #ifndef TSU_WAVES_FP64
 #define TSU_WAVES_FP64 2
 #endif
