@@ -586,8 +586,8 @@ __device__ __forceinline__ void from_normal(int kind, R a, R b, R& fx, R& fy)
 // order) already computed, which the FP64 path recomputes from h u and h c.
 // With SED false the bed and concentration rows are identically zero (no
 // bedload, no suspension, hc == 0) and are not computed.
-// FB (FP64 with sediment, fast_bedload(P) true): bedload by bedload_am_pair
-// among the HLL arithmetic; otherwise by bedload_pair with the bed terms.
+// FB (sediment, fast_bedload(P) true): bedload by bedload_am_pair among the
+// HLL arithmetic; otherwise by bedload_pair with the bed terms.
 template <typename R, bool SED, bool FB = false>
 __device__ __forceinline__ void riemann(const DevParams& P, bool bed_mobile,
                                         R hl, R zl, R qnl, R qtl, R hcl,

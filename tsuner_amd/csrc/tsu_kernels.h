@@ -30,8 +30,9 @@ struct StageArgs {
 };
 
 // precision: 0 = FP64 planes, 1 = FP32 planes (FP64 arithmetic).  fast_bed:
-// the run constants satisfy fast_bedload (tsu_device.h); FP64 with sediment
-// then takes the kernel that evaluates bedload among the HLL arithmetic.
+// the run constants satisfy fast_bedload (tsu_device.h); with sediment the
+// launch then takes the kernel that evaluates bedload among the HLL
+// arithmetic.
 hipError_t launch_stage(const StageArgs& a, bool sed, int precision, hipStream_t s, bool fast_bed = false);
 
 // Ensemble member as seen by the batched stage kernel: byte offsets into

@@ -297,7 +297,7 @@ __device__ __forceinline__ void edge_flux(const DevParams& P, int kind, bool mob
     }
 }
 
-// FB: the run constants satisfy fast_bedload (FP64 with sediment only).
+// FB: the run constants satisfy fast_bedload (kernels with sediment only).
 template <typename R, typename S, bool SED, bool FB>
 __device__ __forceinline__ void stage_body(const StageArgs& a, const int tile)
 {
