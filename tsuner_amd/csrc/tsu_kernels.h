@@ -29,7 +29,7 @@ struct StageArgs {
     const DevParams* P;   // run constants in device memory
 };
 
-// precision: 0 = FP64 planes, 1 = FP32 planes (FP64 arithmetic).  fast_bed:
+// precision: 0 = FP64, 1 = FP32 planes and arithmetic (compensated h, H, hc).  fast_bed:
 // the run constants satisfy fast_bedload (tsu_device.h); with sediment the
 // launch then takes the kernel that evaluates bedload among the HLL
 // arithmetic.
