@@ -302,9 +302,18 @@ template <typename R, typename S, bool SED, bool FB>
 __device__ __forceinline__ void stage_body(const StageArgs& a, const int tile)
 {
     constexpr bool COMP = sizeof(R) == 4;   // FP32 solver: compensated h, H, hc
-    // Run constants are read through a pointer: fields used only in rare
-    // branches are then loaded there instead of pinning scalar registers.
-    const DevParams& P = *a.P;
+    // Run constants, copied into LDS once per workgroup (one wave).  Read
+    // through the global pointer, every use in the row loop was a vector
+    // load, since the kernel's own stores might alias the constants; in LDS
+    // they cannot, and a use is a short LDS read.  The loop re-reads them
+    // per row (see there) rather than holding them in registers.
+    __shared__ DevParams Ps;
+    static_assert(sizeof(DevParams) % 8 == 0 && sizeof(DevParams) <= 64 * 8, "one 8-byte word per lane");
+    if (threadIdx.x < sizeof(DevParams) / 8)
+        reinterpret_cast<unsigned long long*>(&Ps)[threadIdx.x] =
+            reinterpret_cast<const unsigned long long*>(a.P)[threadIdx.x];
+    __syncthreads();
+    const DevParams& P = Ps;
     const int lane = threadIdx.x;
     const int strip = tile % a.nstrips, chunk = tile / a.nstrips;
 
@@ -391,6 +400,13 @@ __device__ __forceinline__ void stage_body(const StageArgs& a, const int tile)
 
     // ---- march over the rows of the tile -----------------------------------
     for (int r = y0; r < y1; ++r) {
+        // the run constants' LDS address passed through an empty asm per
+        // row, so that the compiler reads them where they are used instead
+        // of hoisting all of them into registers ahead of the loop
+        typedef const __attribute__((address_space(3))) DevParams* LdsParams;
+        LdsParams Pl = (LdsParams)&Ps;
+        asm volatile("" : "+s"(Pl));
+        const DevParams& P = *(const DevParams*)Pl;
         // next row (r + 2): issued first, consumed after the x work below
         Cell<R> R2;
         load_cell<R, S, SED>(in, Hin, nv, row_off(r + 2), R2);
