@@ -588,46 +588,116 @@ __device__ __forceinline__ void from_normal(int kind, R a, R b, R& fx, R& fy)
 // bedload, no suspension, hc == 0) and are not computed.
 // FB (sediment, fast_bedload(P) true): bedload by bedload_am_pair among the
 // HLL arithmetic; otherwise by bedload_pair with the bed terms.
-template <typename R, bool SED, bool FB = false>
-__device__ __forceinline__ void riemann(const DevParams& P, bool bed_mobile,
-                                        R hl, R zl, R qnl, R qtl, R hcl,
-                                        R hr, R zr, R qnr, R qtr, R hcr,
-                                        R* Dm, R* Dp)
+//
+// The solve is written in phases (rp_*) so that riemann_pair can run two
+// edges' solves side by side in one basic block.
+
+// One edge after its hydrostatic reconstruction: bed levels, depths,
+// velocities and concentrations of both sides.
+template <typename R>
+struct EdgeRP {
+    R zl, zr, h0, h1, c0, c1;
+    Vel<R> u0, u1;
+};
+
+// Hydrostatic depths (VolumeSolver.cu:24-27).
+template <typename R>
+__device__ __forceinline__ void rp_depths(R hl, R zl, R hr, R zr, EdgeRP<R>& e)
+{
+    const R zmH = rmin(zl, zr);
+    e.zl = zl;
+    e.zr = zr;
+    e.h0 = rmax(hl - zl + zmH, (R)0);
+    e.h1 = rmax(hr - zr + zmH, (R)0);
+}
+
+// Velocities and concentrations of both sides.  Returns true (uniform over
+// the wave) when no lane has a velocity or a concentration.
+template <typename R, bool SED>
+__device__ __forceinline__ bool rp_velocities(R hl, R qnl, R qtl, R hcl, R hr, R qnr, R qtr, R hcr, EdgeRP<R>& e)
+{
+    if constexpr (sizeof(R) == 8) {
+        return velocity_conc_pair<SED>(hl, qnl, qtl, hcl, hr, qnr, qtr, hcr, e.u0, e.c0, e.u1, e.c1);
+    } else {
+        const R zero = (R)0;
+        e.u0 = {qnl, qtl, qnl * qnl + qtl * qtl};
+        e.u1 = {qnr, qtr, qnr * qnr + qtr * qtr};
+        e.c0 = SED ? hcl : zero;
+        e.c1 = SED ? hcr : zero;
+        bool moving = e.u0.x != zero || e.u0.y != zero || e.u1.x != zero || e.u1.y != zero;
+        if (SED) moving = moving || e.c0 != zero || e.c1 != zero;
+        return __builtin_amdgcn_ballot_w64(moving) == 0;
+    }
+}
+
+// A lane at rest (equal hydrostatic depths, no velocity and no
+// concentration on either side) has every flux, jump and bed term of the
+// solve equal to +-0 (the speeds and PVM coefficients stay finite; bedload
+// is 0 at u = 0), so Dm = Dp = 0: when no lane of the wave moves (quiet)
+// and no lane has a depth jump, the solve is skipped.
+template <typename R>
+__device__ __forceinline__ bool rp_has_jump(const EdgeRP<R>& e) { return e.h0 != e.h1; }
+
+// Square roots of the depths and the HLL speed bounds before the Roe
+// average (VolumeSolver.cu:63-73, SedFlux.cuh:181-189).
+template <typename R>
+struct RPSpeeds {
+    R sh0, sh1, SL, SR, cm;
+};
+
+// FP64: every square-root argument is 0 or at least 2^-767 unless a depth
+// is below it (g hm >= 4.9 min(h0, h1) when both are wet); a wave with no
+// such lane takes the square roots without their scaling.  FP32 has one
+// form (UNSCALED is ignored).
+template <typename R>
+__device__ __forceinline__ bool rp_needs_scaling(const EdgeRP<R>& e)
+{
+    if constexpr (sizeof(R) == 8)
+        return sqrt_needs_scaling(e.h0) || sqrt_needs_scaling(e.h1);
+    else
+        return false;
+}
+
+template <typename R, bool UNSCALED>
+__device__ __forceinline__ void rp_speeds(const EdgeRP<R>& e, RPSpeeds<R>& s)
+{
+    const R g = (R)kG;
+    const R hm = (R)0.5 * (e.h0 + e.h1);
+    if constexpr (sizeof(R) == 8) {
+        if constexpr (UNSCALED) {
+            s.sh0 = sqrt_unscaled0(e.h0);
+            s.sh1 = sqrt_unscaled0(e.h1);
+            s.SL = e.u0.x - sqrt_unscaled0(g * e.h0);
+            s.SR = e.u1.x + sqrt_unscaled0(g * e.h1);
+            s.cm = sqrt_unscaled0(g * hm);
+        } else {
+            s.sh0 = rsqrt_(e.h0);
+            s.sh1 = rsqrt_(e.h1);
+            s.SL = e.u0.x - rsqrt_(g * e.h0);
+            s.SR = e.u1.x + rsqrt_(g * e.h1);
+            s.cm = rsqrt_(g * hm);
+        }
+    } else {
+        s.sh0 = rsqrt_(e.h0);
+        s.sh1 = rsqrt_(e.h1);
+        // FP32 solver: sqrt(g h) as sqrt(g) sqrt(h), reusing the Roe weights
+        const R sg = (R)kSqrtG;
+        s.SL = e.u0.x - sg * s.sh0;
+        s.SR = e.u1.x + sg * s.sh1;
+        s.cm = sg * rsqrt_(hm);
+    }
+}
+
+// The rest of the solve: Roe velocity, PVM coefficients, fluxes, jumps and
+// the bed and suspended-load rows.
+template <typename R, bool SED, bool FB>
+__device__ __forceinline__ void rp_solve(const DevParams& P, bool bed_mobile, const EdgeRP<R>& e,
+                                         const RPSpeeds<R>& sp, R* Dm, R* Dp)
 {
     const R g = (R)kG, half = (R)0.5, zero = (R)0;
-    R zmH = rmin(zl, zr);
-    R h0 = rmax(hl - zl + zmH, zero);
-    R h1 = rmax(hr - zr + zmH, zero);
-    if (h0 == zero && h1 == zero) {
-        // Both reconstructed depths vanish: every flux, jump and bed term
-        // below is a product with h0, h1, their difference or a zero jump,
-        // so Dm = Dp = 0 (the speeds stay finite: |u| <= VMAX).
-        for (int f = 0; f < (SED ? 5 : 3); f++) Dm[f] = Dp[f] = zero;
-        return;
-    }
-    // A lane at rest (equal hydrostatic depths, no velocity and no
-    // concentration on either side) has every flux, jump and bed term below
-    // equal to +-0 (the speeds and PVM coefficients stay finite; bedload is
-    // 0 at u = 0), so Dm = Dp = 0: when no lane of the wave moves, the solve
-    // is skipped.
-    Vel<R> u0, u1;
-    R c0, c1;
-    bool quiet;   // uniform: no lane has a velocity or a concentration
-    if constexpr (sizeof(R) == 8) {
-        quiet = velocity_conc_pair<SED>(hl, qnl, qtl, hcl, hr, qnr, qtr, hcr, u0, c0, u1, c1);
-    } else {
-        u0 = {qnl, qtl, qnl * qnl + qtl * qtl};
-        u1 = {qnr, qtr, qnr * qnr + qtr * qtr};
-        c0 = SED ? hcl : zero;
-        c1 = SED ? hcr : zero;
-        bool moving = u0.x != zero || u0.y != zero || u1.x != zero || u1.y != zero;
-        if (SED) moving = moving || c0 != zero || c1 != zero;
-        quiet = __builtin_amdgcn_ballot_w64(moving) == 0;
-    }
-    if (quiet && __builtin_amdgcn_ballot_w64(h0 != h1) == 0) {
-        for (int f = 0; f < (SED ? 5 : 3); f++) Dm[f] = Dp[f] = zero;
-        return;
-    }
+    const R h0 = e.h0, h1 = e.h1;
+    const Vel<R>& u0 = e.u0;
+    const Vel<R>& u1 = e.u1;
     R q0n = u0.x * h0, q0t = u0.y * h0;
     R q1n = u1.x * h1, q1t = u1.y * h1;
     R deta = h1 - h0;
@@ -635,44 +705,15 @@ __device__ __forceinline__ void riemann(const DevParams& P, bool bed_mobile,
     R F00 = h0 * u0.x, F01 = F00 * u0.x, F02 = F00 * u0.y;
     R F10 = h1 * u1.x, F11 = F10 * u1.x, F12 = F10 * u1.y;
 
-    // Roe average (SedFlux.cuh:181-189) and HLL speeds (VolumeSolver.cu:63-73)
     R hm = half * (h0 + h1);
-    R sh0, sh1;
-    R SL, SR, cm;
-    if constexpr (sizeof(R) == 8) {
-        // every argument is 0 or at least 2^-767 unless a depth is below
-        // it (g hm >= 4.9 min(h0, h1) when both are wet): then the square
-        // roots skip their scaling, for the whole wave
-        if (!wave_any(sqrt_needs_scaling(h0) || sqrt_needs_scaling(h1))) {
-            sh0 = sqrt_unscaled0(h0);
-            sh1 = sqrt_unscaled0(h1);
-            SL = u0.x - sqrt_unscaled0(g * h0);
-            SR = u1.x + sqrt_unscaled0(g * h1);
-            cm = sqrt_unscaled0(g * hm);
-        } else {
-            sh0 = rsqrt_(h0);
-            sh1 = rsqrt_(h1);
-            SL = u0.x - rsqrt_(g * h0);
-            SR = u1.x + rsqrt_(g * h1);
-            cm = rsqrt_(g * hm);
-        }
-    } else {
-        sh0 = rsqrt_(h0);
-        sh1 = rsqrt_(h1);
-        // FP32 solver: sqrt(g h) as sqrt(g) sqrt(h), reusing the Roe weights
-        const R sg = (R)kSqrtG;
-        SL = u0.x - sg * sh0;
-        SR = u1.x + sg * sh1;
-        cm = sg * rsqrt_(hm);
-    }
     R F03 = zero, F13 = zero;
     if constexpr (FB && SED) bedload_am_pair(P, bed_mobile, h0, u0, h1, u1, F03, F13);
     // h0 + h1 > 0 here, so sqrt(h0) + sqrt(h1) > 0: the Roe velocity is
-    // always defined (the reference's 0/0 of two dry sides exits above).
-    R ssum = sh0 + sh1;
-    R umx = rdiv(sh0 * u0.x + sh1 * u1.x, ssum);
-    SL = rmin(SL, umx - cm);
-    SR = rmax(SR, umx + cm);
+    // always defined (the reference's 0/0 of two dry sides exits before).
+    R ssum = sp.sh0 + sp.sh1;
+    R umx = rdiv(sp.sh0 * u0.x + sp.sh1 * u1.x, ssum);
+    R SL = rmin(sp.SL, umx - sp.cm);
+    R SR = rmax(sp.SR, umx + sp.cm);
 
     // PVM coefficients; both branches of VolumeSolver.cu:75-83 evaluated and
     // selected (no block split), the division on a denominator kept in range.
@@ -708,12 +749,12 @@ __device__ __forceinline__ void riemann(const DevParams& P, bool bed_mobile,
     Dp[2] = rmad(half, df2 + v2, -F12);
 
     if (SED) {
-        R hc0 = h0 * c0, hc1 = h1 * c1;
+        R hc0 = h0 * e.c0, hc1 = h1 * e.c1;
         if constexpr (!FB) bedload_pair(P, bed_mobile, h0, u0, h1, u1, F03, F13);
         R F04 = hc0 * u0.x, F14 = hc1 * u1.x;
         R df3 = F13 - F03, df4 = F14 - F04;
         R DU4 = hc1 - hc0;
-        int s0 = sgn(zr - zl), s1 = sgn(df3), s2 = sgn(a1);
+        int s0 = sgn(e.zr - e.zl), s1 = sgn(df3), s2 = sgn(a1);
         s1 = (s0 != 0) ? s1 * s0 : 0;
         R v3 = (s2 == s1) ? a1 * df3 : -a1 * df3;
         R v4 = rmad(a0, DU4, a1 * df4);
@@ -727,6 +768,77 @@ __device__ __forceinline__ void riemann(const DevParams& P, bool bed_mobile,
             Dp[4] = -Dm[4];
         }
     }
+}
+
+template <typename R, bool SED, bool FB = false>
+__device__ __forceinline__ void riemann(const DevParams& P, bool bed_mobile,
+                                        R hl, R zl, R qnl, R qtl, R hcl,
+                                        R hr, R zr, R qnr, R qtr, R hcr,
+                                        R* Dm, R* Dp)
+{
+    EdgeRP<R> e;
+    rp_depths(hl, zl, hr, zr, e);
+    if (e.h0 == (R)0 && e.h1 == (R)0) {
+        // Both reconstructed depths vanish: every flux, jump and bed term
+        // is a product with h0, h1, their difference or a zero jump, so
+        // Dm = Dp = 0 (the speeds stay finite: |u| <= VMAX).
+        for (int f = 0; f < (SED ? 5 : 3); f++) Dm[f] = Dp[f] = (R)0;
+        return;
+    }
+    const bool quiet = rp_velocities<R, SED>(hl, qnl, qtl, hcl, hr, qnr, qtr, hcr, e);
+    if (quiet && __builtin_amdgcn_ballot_w64(rp_has_jump(e)) == 0) {
+        for (int f = 0; f < (SED ? 5 : 3); f++) Dm[f] = Dp[f] = (R)0;
+        return;
+    }
+    RPSpeeds<R> sp;
+    if (sizeof(R) == 4 || !wave_any(rp_needs_scaling(e)))
+        rp_speeds<R, true>(e, sp);
+    else
+        rp_speeds<R, false>(e, sp);
+    rp_solve<R, SED, FB>(P, bed_mobile, e, sp, Dm, Dp);
+}
+
+// Two edges' Riemann problems (values identical to two riemann calls):
+// side[i] = (hl, zl, qnl, qtl, hcl, hr, zr, qnr, qtr, hcr) of edge i.  The
+// wave-wide tests are taken over both edges, so that the two solves run
+// side by side in one basic block and their dependency chains overlap; a
+// lane whose two depths vanish on an edge gets that edge's zeros by a
+// select instead of an early exit.
+template <typename R, bool SED, bool FB>
+__device__ __forceinline__ void riemann_pair(const DevParams& P, const bool (&mobile)[2], const R (&side)[2][10],
+                                             R (&Dm)[2][5], R (&Dp)[2][5])
+{
+    constexpr int NF = SED ? 5 : 3;
+    EdgeRP<R> e[2];
+    bool dry[2], quiet = true;
+    for (int i = 0; i < 2; i++) {
+        rp_depths(side[i][0], side[i][1], side[i][5], side[i][6], e[i]);
+        dry[i] = e[i].h0 == (R)0 && e[i].h1 == (R)0;
+    }
+    for (int i = 0; i < 2; i++) {
+        const bool q = rp_velocities<R, SED>(side[i][0], side[i][2], side[i][3], side[i][4], side[i][5], side[i][7],
+                                             side[i][8], side[i][9], e[i]);
+        quiet = quiet && q;
+    }
+    if (quiet && __builtin_amdgcn_ballot_w64(rp_has_jump(e[0]) || rp_has_jump(e[1])) == 0) {
+        for (int i = 0; i < 2; i++)
+            for (int f = 0; f < NF; f++) Dm[i][f] = Dp[i][f] = (R)0;
+        return;
+    }
+    RPSpeeds<R> sp[2];
+    if (sizeof(R) == 4 || !wave_any(rp_needs_scaling(e[0]) || rp_needs_scaling(e[1]))) {
+        rp_speeds<R, true>(e[0], sp[0]);
+        rp_speeds<R, true>(e[1], sp[1]);
+    } else {
+        rp_speeds<R, false>(e[0], sp[0]);
+        rp_speeds<R, false>(e[1], sp[1]);
+    }
+    for (int i = 0; i < 2; i++) rp_solve<R, SED, FB>(P, mobile[i], e[i], sp[i], Dm[i], Dp[i]);
+    for (int i = 0; i < 2; i++)
+        for (int f = 0; f < NF; f++) {
+            Dm[i][f] = dry[i] ? (R)0 : Dm[i][f];
+            Dp[i][f] = dry[i] ? (R)0 : Dp[i][f];
+        }
 }
 
 // Reconstructor.cu:57-88 with SedFlux.cuh:51-99: exchange rate phi between

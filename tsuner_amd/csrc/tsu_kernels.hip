@@ -297,6 +297,43 @@ __device__ __forceinline__ void edge_flux(const DevParams& P, int kind, bool mob
     }
 }
 
+// Two edges' Riemann problems side by side (riemann_pair), each edge as
+// edge_flux: edge a (kind ka, bed mobility ma, faces La/Ra, fluctuations
+// Dma/Dpa) and edge b likewise.
+template <typename R, bool SED, bool FB>
+__device__ __forceinline__ void edge_flux_pair(const DevParams& P, int ka, bool ma, const R* La, const R* Ra, R* Dma,
+                                               R* Dpa, int kb, bool mb, const R* Lb, const R* Rb, R* Dmb, R* Dpb)
+{
+    const int kind[2] = {ka, kb};
+    const bool mobile[2] = {ma, mb};
+    const R* const L[2] = {La, Lb};
+    const R* const Rt[2] = {Ra, Rb};
+    R* const Dm[2] = {Dma, Dmb};
+    R* const Dp[2] = {Dpa, Dpb};
+    R side[2][10], dm[2][5], dp[2][5];
+    for (int i = 0; i < 2; i++) {
+        side[i][0] = L[i][0];
+        side[i][1] = L[i][3];
+        to_normal(kind[i], L[i][1], L[i][2], side[i][2], side[i][3]);
+        side[i][4] = L[i][4];
+        side[i][5] = Rt[i][0];
+        side[i][6] = Rt[i][3];
+        to_normal(kind[i], Rt[i][1], Rt[i][2], side[i][7], side[i][8]);
+        side[i][9] = Rt[i][4];
+    }
+    riemann_pair<R, SED, FB>(P, mobile, side, dm, dp);
+    for (int i = 0; i < 2; i++) {
+        Dm[i][0] = dm[i][0];
+        Dp[i][0] = dp[i][0];
+        from_normal(kind[i], dm[i][1], dm[i][2], Dm[i][1], Dm[i][2]);
+        from_normal(kind[i], dp[i][1], dp[i][2], Dp[i][1], Dp[i][2]);
+        for (int f = 3; f < 5; f++) {
+            Dm[i][f] = SED ? dm[i][f] : (R)0;
+            Dp[i][f] = SED ? dp[i][f] : (R)0;
+        }
+    }
+}
+
 // FB: the run constants satisfy fast_bedload (kernels with sediment only).
 template <typename R, typename S, bool SED, bool FB>
 __device__ __forceinline__ void stage_body(const StageArgs& a, const int tile)
@@ -478,14 +515,6 @@ __device__ __forceinline__ void stage_body(const StageArgs& a, const int tile)
             }
             fixed = (perx && (x == 0 || x == npx)) ? fxl : fxl * fx0;
         }
-        R Dmx[5], Dpx[5];
-        edge_flux<R, SED, FB>(P, kind, fixed != 0, El, Er, Dmx, Dpx);
-        R FL[5], FR[5];
-        const bool left_wall = (x == 0 && !perx);
-        for (int f = 0; f < 5; f++) {
-            FL[f] = left_wall ? Dmx[f] : Dpx[f];
-            FR[f] = dn1(Dmx[f]);
-        }
 
         // y faces of row r + 1 and the edge r + 1/2
         derive_cell<R, SED>(R2);
@@ -498,14 +527,23 @@ __device__ __forceinline__ void stage_body(const StageArgs& a, const int tile)
             cell_face(R1, D1);
             cell_face(R1, U1);
         }
-        R FU0[5], FD1[5];
-        if (r + 1 == npy && !pery) {
-            R G5[5], dp[5];
-            ghost_state(P, 3, U0, G5, (long long)npx + npy + xc, 2, ct);
-            edge_flux<R, SED, FB>(P, N_PY, fx0 != 0, U0, G5, FU0, dp);
-        } else {
-            int fy = (r + 1 == npy) ? fx0 : fx0 * fx1;
-            edge_flux<R, SED, FB>(P, N_PY, fy != 0, U0, D1, FU0, FD1);
+        // upper side of the edge r + 1/2: row r + 1's lower faces, or the
+        // ghost state above the domain's last row
+        R Yr[5];
+        if (r + 1 == npy && !pery)
+            ghost_state(P, 3, U0, Yr, (long long)npx + npy + xc, 2, ct);
+        else
+            for (int f = 0; f < 5; f++) Yr[f] = D1[f];
+        const int fy = (r + 1 == npy) ? fx0 : fx0 * fx1;
+
+        // the x edge x - 1/2 and the y edge r + 1/2, solved side by side
+        R Dmx[5], Dpx[5], FU0[5], FD1[5];   // FD1: not used after the domain's last row
+        edge_flux_pair<R, SED, FB>(P, kind, fixed != 0, El, Er, Dmx, Dpx, N_PY, fy != 0, U0, Yr, FU0, FD1);
+        R FL[5], FR[5];
+        const bool left_wall = (x == 0 && !perx);
+        for (int f = 0; f < 5; f++) {
+            FL[f] = left_wall ? Dmx[f] : Dpx[f];
+            FR[f] = dn1(Dmx[f]);
         }
 
         // cell update of row r (Reconstructor.cu:197,274,353-354;
