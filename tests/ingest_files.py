@@ -50,6 +50,135 @@ def write_series(stem, lon, lat, times, h, u, v, upper=True, version=1):
     return paths
 
 
+# NetCDF classic files written from the published format description, in
+# any of its versions (scipy writes only 1 and 2): big-endian, magic "CDF" +
+# version byte; counts, lengths, dimension ids and vsize 32-bit in versions
+# 1 and 2, 64-bit in version 5 (the "64-bit data" format, CDF-5); offsets
+# 32-bit in version 1, 64-bit in 2 and 5; tags and type codes 32-bit; names
+# and values padded to 4 bytes; record variables interleaved record by
+# record (unpadded when there is only one).
+_NC_TYPES = {"i1": 1, "S1": 2, "i2": 3, "i4": 4, "f4": 5, "f8": 6,
+             "u1": 7, "u2": 8, "u4": 9, "i8": 10, "u8": 11}
+# default fill values, which pad variable data to 4 bytes
+_NC_FILL = {"i1": -127, "S1": 0, "i2": -32767, "u1": 255, "u2": 65535}
+
+
+def write_classic(path, version, dims, variables, gatts=None):
+    """dims: [(name, length or None for the record dimension)];
+    variables: [(name, (dim names), dtype string, array, {attr: value})];
+    gatts: {name: str or array}."""
+    import struct
+    nfmt = ">q" if version == 5 else ">i"
+    ofmt = ">i" if version == 1 else ">q"
+
+    def i4(v): return struct.pack(">i", v)
+    def nn(v): return struct.pack(nfmt, v)
+
+    def pad(b): return b + b"\0" * (-len(b) % 4)
+
+    def pad_data(b, dt):
+        n = (-len(b) % 4) // np.dtype(dt).itemsize
+        return b + np.full(n, _NC_FILL.get(dt, 0), ">" + dt).tobytes()
+
+    def name(n):
+        e = n.encode()
+        return nn(len(e)) + pad(e)
+
+    def atts(d):
+        if not d:
+            return i4(0) + nn(0)
+        out = i4(0x0C) + nn(len(d))
+        for k, v in d.items():
+            if isinstance(v, str):
+                raw, t, n = v.encode(), 2, len(v.encode())
+            else:
+                a = np.asarray(v)
+                dt = a.dtype.str[1:]
+                raw, t, n = a.astype(">" + dt).tobytes(), _NC_TYPES[dt], a.size
+            out += name(k) + i4(t) + nn(n) + pad(raw)
+        return out
+
+    dlen = {n: l for n, l in dims}
+    rec_dim = next((n for n, l in dims if l is None), None)
+    is_rec = [bool(v[1]) and v[1][0] == rec_dim for v in variables]
+    recs = [v for v, r in zip(variables, is_rec) if r]
+    numrecs = max((len(v[3]) for v in recs), default=0)
+    dim_list = (i4(0x0A) + nn(len(dims)) + b"".join(name(n) + nn(l or 0) for n, l in dims)) if dims \
+        else i4(0) + nn(0)
+
+    def slab(v, rec):   # bytes of the whole variable, or of one record
+        shape = [dlen[d] for d in v[1][(1 if rec else 0):]]
+        return int(np.prod(shape, dtype=np.int64)) * np.dtype(v[2]).itemsize
+
+    def vsize(v, rec): return slab(v, rec) + (-slab(v, rec) % 4)
+
+    def header(begins):
+        out = b"CDF" + bytes([version]) + nn(numrecs) + dim_list + atts(gatts)
+        out += (i4(0x0B) + nn(len(variables))) if variables else i4(0) + nn(0)
+        for v, rec, beg in zip(variables, is_rec, begins):
+            out += name(v[0]) + nn(len(v[1])) + b"".join(nn([n for n, _ in dims].index(d)) for d in v[1])
+            out += atts(v[4]) + i4(_NC_TYPES[v[2]]) + nn(vsize(v, rec)) + struct.pack(ofmt, beg)
+        return out
+
+    off = len(header([0] * len(variables)))
+    begins = []
+    for v, rec in zip(variables, is_rec):
+        begins.append(None if rec else off)
+        off += 0 if rec else vsize(v, False)
+    rec_start = off
+    for i, (v, rec) in enumerate(zip(variables, is_rec)):
+        if rec:
+            begins[i] = off
+            off += vsize(v, True)
+    data = bytearray(header(begins))
+    for v, rec in zip(variables, is_rec):
+        if not rec:
+            data += pad_data(np.ascontiguousarray(v[3]).astype(">" + v[2]).tobytes(), v[2])
+    for r in range(numrecs):
+        for v in recs:
+            raw = np.ascontiguousarray(v[3][r]).astype(">" + v[2]).tobytes()
+            data += raw if len(recs) == 1 else pad_data(raw, v[2])
+    with open(path, "wb") as f:
+        f.write(bytes(data))
+
+
+def scipy_order(dims, variables):
+    """The variables in the order scipy's writer lays them out: non-record
+    variables by shape, largest first (stable), then the record variables."""
+    rec = next((n for n, l in dims if l is None), None)
+    dl = dict(dims)
+    key = lambda v: (-1,) if v[1] and v[1][0] == rec else tuple(dl[d] for d in v[1])
+    return sorted(variables, key=key, reverse=True)
+
+
+def write_bathy_classic(path, lon, lat, bathy, var_dtype="f4", coord_names=("lon", "lat"), version=5):
+    """write_bathy's file, here in any classic version (default CDF-5)."""
+    x, y = coord_names
+    dims = [(x, len(lon)), (y, len(lat))]
+    write_classic(path, version, dims, scipy_order(dims, [
+        (x, (x,), "f8", np.asarray(lon, np.float64), {}),
+        (y, (y,), "f8", np.asarray(lat, np.float64), {}),
+        ("bathy", (y, x), var_dtype, np.asarray(bathy), {"units": "meters", "positive": "down"})]))
+
+
+def write_series_classic(stem, lon, lat, times, h, u, v, upper=True, version=5, gatts=None):
+    """write_series' files, here in any classic version (default CDF-5)."""
+    names = dict(lon="LON", lat="LAT", time="TIME", h="HA", u="UA", v="VA") if upper else \
+        dict(lon="lon", lat="lat", time="time", h="ha_lc", u="ua_lc", v="va_lc")
+    paths = []
+    for c, data in (("h", h), ("u", u), ("v", v)):
+        path = str(stem) + c + ".nc"
+        T, X, Y = names["time"], names["lon"], names["lat"]
+        dims = [(T, None if upper else len(times)), (Y, len(lat)), (X, len(lon))]
+        write_classic(path, version, dims, scipy_order(dims, [
+            (T, (T,), "f4", np.asarray(times, np.float32), {}),
+            (X, (X,), "f4", np.asarray(lon, np.float32), {}),
+            (Y, (Y,), "f4", np.asarray(lat, np.float32), {}),
+            (names[c], (T, Y, X), "f4", np.asarray(data, np.float32), {})]), gatts=gatts)
+        paths.append(path)
+    return paths
+
+
 def coastal_case(tmp, nlon=90, nlat=70, T=8, seed=0, fill=False, upper=True):
     """A sloping coast near 46.9 N: depth 40 m offshore (west) to 8 m of land
     (east) with seeded bumps; a free-surface series of a long wave entering

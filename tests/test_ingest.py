@@ -13,7 +13,8 @@ import pyingest as OI
 from tsuner_amd import _lib, ingest
 from tsuner_amd import scenarios as S
 
-from ingest_files import coastal_case, write_bathy, write_series, write_series_hdf5
+from ingest_files import (coastal_case, write_bathy, write_bathy_classic, write_classic, write_series,
+                          write_series_classic, write_series_hdf5)
 
 EXAMPLE = "/root/reference/example"
 
@@ -192,6 +193,102 @@ def test_library_matches_oracle_on_hdf5_series(tmp_path):
     compare(h5, OI.load_problem_files(b, paths[0], "0", 0.05, 70, 55))
     for k in ("h", "H", "CC", "tCC"):
         np.testing.assert_array_equal(h5[k], classic[k], err_msg=k)
+
+
+def test_classic_writer_matches_scipy(tmp_path):
+    """The test suite's own classic-format writer (the only one here that
+    writes CDF-5) gives scipy's bytes for versions 1 and 2: header layout,
+    variable order, padding and record interleaving are those of an
+    independent writer, and version 5 changes only the magic byte and the
+    field widths."""
+    rng = np.random.default_rng(4)
+    lon, lat = np.linspace(235.8, 235.9, 7), np.linspace(46.8, 46.9, 5)
+    t = np.array([30.0, 60.0, 90.0])
+    h = rng.normal(size=(3, 5, 7)).astype(np.float32)
+    for version in (1, 2):
+        for dt in ("f4", "f8", "i1", "i2", "i4"):
+            data = (rng.normal(size=(5, 7)) * 50).astype(dt)
+            write_bathy(tmp_path / "a.nc", lon, lat, data, var_dtype=dt, coord_names=("x", "y"), version=version)
+            write_bathy_classic(tmp_path / "b.nc", lon, lat, data, var_dtype=dt, coord_names=("x", "y"),
+                                version=version)
+            assert (tmp_path / "a.nc").read_bytes() == (tmp_path / "b.nc").read_bytes(), (version, dt)
+        for upper in (True, False):
+            p1 = write_series(tmp_path / "s1_", lon, lat, t, h, h + 1, h + 2, upper=upper, version=version)
+            p2 = write_series_classic(tmp_path / "s2_", lon, lat, t, h, h + 1, h + 2, upper=upper, version=version)
+            for x, y in zip(p1, p2):
+                assert open(x, "rb").read() == open(y, "rb").read(), (version, upper, x)
+
+
+def _read_classic(path, names):
+    from scipy.io import netcdf_file
+    with netcdf_file(path, "r", mmap=False) as f:
+        return [np.array(f.variables[k].data) for k in names]
+
+
+@pytest.mark.parametrize("upper", [True, False])
+def test_cdf5_files_load_like_classic(tmp_path, upper):
+    """The 64-bit data format (CDF-5: 64-bit counts, lengths and ids, record
+    variables, global attributes of 64-bit integers) reads to the same arrays
+    as the classic files it copies."""
+    b, e = coastal_case(tmp_path, seed=21, T=5, upper=upper)
+    lon, lat, bathy = _read_classic(b, ("lon", "lat", "bathy"))
+    b5 = str(tmp_path / "b5.nc")
+    write_bathy_classic(b5, lon, lat, bathy)
+    k = ("LON", "LAT", "TIME") if upper else ("lon", "lat", "time")
+    v = ("HA", "UA", "VA") if upper else ("ha_lc", "ua_lc", "va_lc")
+    slon, slat, t = _read_classic(e, k)
+    series = [_read_classic(e.replace("h.nc", c + ".nc"), (n,))[0] for c, n in zip("huv", v)]
+    paths = write_series_classic(str(tmp_path / "c5_"), slon, slat, t, *series, upper=upper,
+                                 gatts={"title": "CDF-5 copy", "levels": np.arange(3, dtype=np.int64)})
+    for p in (b5, paths[0]):
+        with open(p, "rb") as f:
+            assert f.read(4) == b"CDF\x05"
+    L = _lib.lib()
+    T = C.c_ulong(0)
+    assert L.tsu_input_time_levels(paths[0].encode(), C.byref(T)) == 0 and T.value == 5
+    ref = ingest.load_input_files(b, e, params(60, 45), sea_level_correction=0.1)
+    cdf5 = ingest.load_input_files(b5, paths[0], params(60, 45), sea_level_correction=0.1)
+    for key in ("h", "H", "CC", "tCC", "fixH"):
+        np.testing.assert_array_equal(cdf5[key], ref[key], err_msg=key)
+
+
+@pytest.mark.parametrize("dt5,dt1", [("i8", "f8"), ("u2", "f8"), ("u4", "f8"), ("u8", "f8")])
+def test_cdf5_integer_types(tmp_path, dt5, dt1):
+    """The integer types that only version 5 has: a bathymetry of whole
+    metres stored as such loads like the same values as doubles in a
+    version-1 file, and a fixed-bed map of unsigned bytes like short
+    integers."""
+    b, e = coastal_case(tmp_path, seed=8, T=3)
+    lon, lat, bathy = _read_classic(b, ("lon", "lat", "bathy"))
+    whole = np.round(bathy + 30.0).astype(np.float64)   # non-negative whole metres, for the unsigned types
+    write_bathy(tmp_path / "b1.nc", lon, lat, whole, var_dtype=dt1)
+    write_bathy_classic(tmp_path / "b5.nc", lon, lat, whole.astype(dt5))
+    mlon = 235.78 + 0.004 * np.arange(40)
+    mlat = 46.84 + 0.003 * np.arange(35)
+    mask = np.ones((35, 40), np.int16)
+    mask[8:22, 6:31] = 0
+    write_bathy(tmp_path / "f1.nc", mlon, mlat, mask, var_dtype="i2", coord_names=("x", "y"), version=2)
+    write_bathy_classic(tmp_path / "f5.nc", mlon, mlat, mask.astype(np.uint8), var_dtype="u1",
+                        coord_names=("x", "y"))
+    one = ingest.load_input_files(str(tmp_path / "b1.nc"), e, params(48, 36),
+                                  fixed_bed_file=str(tmp_path / "f1.nc"))
+    five = ingest.load_input_files(str(tmp_path / "b5.nc"), e, params(48, 36),
+                                   fixed_bed_file=str(tmp_path / "f5.nc"))
+    for key in ("h", "H", "CC", "tCC", "fixH"):
+        np.testing.assert_array_equal(five[key], one[key], err_msg=key)
+    assert 0 < (five["fixH"] == 0).sum() < five["fixH"].size
+
+
+def test_version5_types_rejected_in_classic_versions(tmp_path):
+    """Type codes 7-11 exist only in version 5: a version-1 or -2 header that
+    uses one is malformed."""
+    L = _lib.lib()
+    T = C.c_ulong(0)
+    for version in (1, 2):
+        path = tmp_path / ("bad%d_h.nc" % version)
+        write_classic(path, version, [("TIME", None)], [("TIME", ("TIME",), "i8", np.arange(3, dtype=np.int64), {})])
+        assert L.tsu_input_time_levels(str(path).encode(), C.byref(T)) == 2
+        assert b"malformed" in L.tsu_last_error()
 
 
 def test_longer_velocity_series_use_their_first_levels(tmp_path):

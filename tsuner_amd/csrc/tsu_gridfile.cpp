@@ -2,9 +2,13 @@
 // behind tsu::GridFile (declared in tsu_gridfile.h).
 //
 // Classic format (big-endian): magic "CDF" + version byte (1: 32-bit
-// offsets, 2: 64-bit offsets), numrecs, then the dimension, global
-// attribute and variable lists; a list is ABSENT (two zero words) or a tag
-// (0x0A dimensions, 0x0C attributes, 0x0B variables) and a count.  Names and
+// offsets, 2: 64-bit offsets, 5: 64-bit offsets and 64-bit counts, the
+// "64-bit data" format), numrecs, then the dimension, global attribute and
+// variable lists; a list is ABSENT (a zero tag and a zero count) or a tag
+// (0x0A dimensions, 0x0C attributes, 0x0B variables) and a count.  Counts,
+// name lengths, dimension lengths and ids, vsize and numrecs are 32-bit
+// (versions 1 and 2) or 64-bit (version 5); type codes and tags stay
+// 32-bit.  Version 5 adds the unsigned and 64-bit integer types.  Names and
 // attribute values are padded to 4 bytes.  A variable whose first dimension
 // has length 0 is a record variable: record r of it starts at
 // begin + r*recsize, recsize being the sum of the record variables' vsize
@@ -23,17 +27,26 @@ namespace tsu {
 
 namespace {
 
-enum { NC_BYTE = 1, NC_CHAR = 2, NC_SHORT = 3, NC_INT = 4, NC_FLOAT = 5, NC_DOUBLE = 6 };
+enum {
+    NC_BYTE = 1, NC_CHAR = 2, NC_SHORT = 3, NC_INT = 4, NC_FLOAT = 5, NC_DOUBLE = 6,
+    // version 5 only
+    NC_UBYTE = 7, NC_USHORT = 8, NC_UINT = 9, NC_INT64 = 10, NC_UINT64 = 11
+};
 
 int type_size(int t)
 {
     switch (t) {
     case NC_BYTE:
-    case NC_CHAR: return 1;
-    case NC_SHORT: return 2;
+    case NC_CHAR:
+    case NC_UBYTE: return 1;
+    case NC_SHORT:
+    case NC_USHORT: return 2;
     case NC_INT:
-    case NC_FLOAT: return 4;
-    case NC_DOUBLE: return 8;
+    case NC_FLOAT:
+    case NC_UINT: return 4;
+    case NC_DOUBLE:
+    case NC_INT64:
+    case NC_UINT64: return 8;
     }
     return 0;
 }
@@ -53,7 +66,36 @@ void decode_all(const uint8_t* raw, size_t n, int t, double* out)
         for (size_t i = 0; i < n; i++) out[i] = (double)(int8_t)raw[i];
         break;
     case NC_CHAR:
+    case NC_UBYTE:
         for (size_t i = 0; i < n; i++) out[i] = (double)raw[i];
+        break;
+    case NC_USHORT:
+        for (size_t i = 0; i < n; i++) {
+            uint16_t u;
+            std::memcpy(&u, raw + 2 * i, 2);
+            out[i] = (double)__builtin_bswap16(u);
+        }
+        break;
+    case NC_UINT:
+        for (size_t i = 0; i < n; i++) {
+            uint32_t u;
+            std::memcpy(&u, raw + 4 * i, 4);
+            out[i] = (double)__builtin_bswap32(u);
+        }
+        break;
+    case NC_INT64:
+        for (size_t i = 0; i < n; i++) {
+            uint64_t u;
+            std::memcpy(&u, raw + 8 * i, 8);
+            out[i] = (double)(int64_t)__builtin_bswap64(u);
+        }
+        break;
+    case NC_UINT64:
+        for (size_t i = 0; i < n; i++) {
+            uint64_t u;
+            std::memcpy(&u, raw + 8 * i, 8);
+            out[i] = (double)__builtin_bswap64(u);
+        }
         break;
     case NC_SHORT:
         for (size_t i = 0; i < n; i++) {
@@ -93,6 +135,8 @@ void decode_all(const uint8_t* raw, size_t n, int t, double* out)
 // Forward-only cursor over the file, reading more as the header needs it.
 struct Cursor {
     std::FILE* f;
+    int nn = 4;   // bytes of a count, length, id or size (8 in version 5)
+    int max_type = 6;
     std::vector<uint8_t> buf;
     size_t pos = 0;
     bool ok = true;
@@ -117,7 +161,7 @@ struct Cursor {
     }
     std::string name()
     {
-        const size_t n = (size_t)u(4);
+        const size_t n = (size_t)u(nn);
         const size_t padded = (n + 3) & ~(size_t)3;
         if (n > (1u << 20) || !need(padded)) {
             ok = false;
@@ -129,7 +173,7 @@ struct Cursor {
     }
     void skip_atts()
     {
-        const uint64_t tag = u(4), count = u(4);
+        const uint64_t tag = u(4), count = u(nn);
         if (tag == 0 && count == 0) return;
         if (tag != 0x0C || count > (1u << 20)) {
             ok = false;
@@ -138,9 +182,9 @@ struct Cursor {
         for (uint64_t a = 0; a < count && ok; a++) {
             name();
             const int t = (int)u(4);
-            const uint64_t n = u(4);
+            const uint64_t n = u(nn);
             const uint64_t bytes = ((uint64_t)type_size(t) * n + 3) & ~(uint64_t)3;
-            if (type_size(t) == 0 || bytes > (1u << 26) || !need((size_t)bytes)) {
+            if (t > max_type || type_size(t) == 0 || n > (1u << 26) || bytes > (1u << 26) || !need((size_t)bytes)) {
                 ok = false;
                 return;
             }
@@ -305,46 +349,51 @@ bool GridFile::open(const std::string& path)
         hdf5_ = true;
         return true;
     }
-    if (got < 4 || std::memcmp(magic, "CDF", 3) != 0 || (magic[3] != 1 && magic[3] != 2)) {
+    const int version = got >= 4 ? magic[3] : 0;
+    if (got < 4 || std::memcmp(magic, "CDF", 3) != 0 || (version != 1 && version != 2 && version != 5)) {
         std::fclose(f);
-        err_ = path + " is neither a NetCDF classic (CDF-1/CDF-2) nor a NetCDF-4/HDF5 file";
+        err_ = path + " is neither a NetCDF classic (CDF-1/CDF-2/CDF-5) nor a NetCDF-4/HDF5 file";
         return false;
     }
-    const int off_bytes = magic[3] == 2 ? 8 : 4;
+    const int off_bytes = version == 1 ? 4 : 8;
     std::fseek(f, 0, SEEK_SET);
     Cursor c{f};
+    c.nn = version == 5 ? 8 : 4;
+    c.max_type = version == 5 ? NC_UINT64 : NC_DOUBLE;
     c.u(4);
-    im_->numrecs = c.u(4);
+    im_->numrecs = c.u(c.nn);
+    const uint64_t streaming = version == 5 ? ~0ull : 0xFFFFFFFFull;
     // dimensions
-    uint64_t tag = c.u(4), count = c.u(4);
+    uint64_t tag = c.u(4), count = c.u(c.nn);
     if (!(tag == 0 && count == 0)) {
         if (tag != 0x0A || count > (1u << 20)) c.ok = false;
         for (uint64_t i = 0; i < count && c.ok; i++) {
             Impl::Dim d;
             d.name = c.name();
-            d.len = c.u(4);
+            d.len = c.u(c.nn);
             im_->dims.push_back(d);
         }
     }
     c.skip_atts();
     tag = c.u(4);
-    count = c.u(4);
+    count = c.u(c.nn);
     if (!(tag == 0 && count == 0)) {
         if (tag != 0x0B || count > (1u << 20)) c.ok = false;
         for (uint64_t i = 0; i < count && c.ok; i++) {
             Impl::Var v;
             v.name = c.name();
-            const uint64_t nd = c.u(4);
-            for (uint64_t k = 0; k < nd && c.ok && k < 1024; k++) {
-                const uint64_t id = c.u(4);
+            const uint64_t nd = c.u(c.nn);
+            if (nd > 1024) c.ok = false;
+            for (uint64_t k = 0; k < nd && c.ok; k++) {
+                const uint64_t id = c.u(c.nn);
                 if (id >= im_->dims.size()) c.ok = false;
                 v.dims.push_back((int)id);
             }
             c.skip_atts();
             v.type = (int)c.u(4);
-            v.vsize = c.u(4);
+            v.vsize = c.u(c.nn);
             v.begin = c.u(off_bytes);
-            if (type_size(v.type) == 0) c.ok = false;
+            if (v.type > c.max_type || type_size(v.type) == 0) c.ok = false;
             v.rec = c.ok && !v.dims.empty() && im_->dims[(size_t)v.dims[0]].len == 0;
             im_->vars.push_back(v);
         }
@@ -367,7 +416,7 @@ bool GridFile::open(const std::string& path)
                 for (size_t k = 1; k < v.dims.size(); k++) n *= im_->dims[(size_t)v.dims[k]].len;
                 im_->recsize = n;
             }
-    if (im_->numrecs == 0xFFFFFFFFull) {   // streaming: count from the file size
+    if (im_->numrecs == streaming) {   // streaming: count from the file size
         std::fseek(f, 0, SEEK_END);
         const uint64_t size = (uint64_t)std::ftell(f);
         uint64_t first = UINT64_MAX;
