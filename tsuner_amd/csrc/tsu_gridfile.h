@@ -1,9 +1,9 @@
 // tsu_gridfile.h -- read-only access to the gridded input files of the
 // real-data path: NetCDF classic (CDF-1, 64-bit-offset CDF-2 and 64-bit-data
 // CDF-5, read with the library's own code from the published format
-// specification) and
-// NetCDF-4 (an HDF5 file: read through the HDF5 C library, loaded at run
-// time so that nothing else in the library depends on it).
+// specification) and NetCDF-4 (an HDF5 file: read through the HDF5 C
+// library, loaded at run time so that nothing else in the library depends
+// on it).
 //
 // The reference opens every input with nc_open and asks for dimensions and
 // variables by name (Problem_loader.cxx:57-175, interpolations.cxx:368-460);
