@@ -341,10 +341,18 @@ def run_correction(heldout, fine_steps):
     head = names[0]
     net, scale_out, info = nets[head]
     others = {k: (nets[k][0], nets[k][1]) for k in names[1:]}
-    ev = K.evaluate(net, scale_out, list(range(300001, 300001 + heldout)), fine_steps=fine_steps, others=others)
+    # the eta_max readout trained on the headline network's runs
+    from tsuner_amd import eta_readout
+    readout = eta_readout.load() if head == "conv4_w32" and os.path.exists(eta_readout.WEIGHTS) else None
+    ev = K.evaluate(net, scale_out, list(range(300001, 300001 + heldout)), fine_steps=fine_steps, others=others,
+                    readout=readout)
     rows = {head: {"coarse_nn": pick(ev["coarse_nn"]),
                    "fallback_policies": {k: pick(v) for k, v in ev["fallback_policies"]["policies"].items()},
                    "network": network(net, info)}}
+    if readout is not None:
+        rows[head]["coarse_nn_eta_readout"] = pick(ev["coarse_nn_readout"])
+        rows[head]["eta_readout_fallback_policies"] = {
+            k: pick(v) for k, v in ev["readout_fallback_policies"]["policies"].items()}
     for k in names[1:]:
         o = ev["others"][k]
         rows[k] = {"coarse_nn": pick(o["coarse_nn"]),
@@ -359,7 +367,9 @@ def run_correction(heldout, fine_steps):
             "metrics": "peak_eta_err: highest eta_max over the nearshore zone (coarse cells initially < 10 m deep, "
                        "land included) vs the fine run's block means; peak_eta_sea_err: the same over the sea part "
                        "of the zone; runup: highest fine ground reached; fallback policies: threshold chosen on the "
-                       "first half of the scenarios (most fallbacks keeping >= 8x there), reported on the second"}
+                       "first half of the scenarios (most fallbacks keeping >= 8x there), reported on the second; "
+                       "coarse_nn_eta_readout: the same run's eta_max read out near the zone peak by the learned "
+                       "per-cell readout (tsuner_amd/eta_readout.py) for the eta_max metrics, runup unchanged"}
 
 
 def calibrate(nbytes, reps=3):

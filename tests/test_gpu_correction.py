@@ -65,12 +65,17 @@ def test_networks_meet_the_bars_and_beat_coarse_alone():
     the scenarios) at >= 8x the fine solver's throughput, each network
     improving on coarse alone in coastal peak, zone peak, nearshore mean and
     runup.  The zone peak (inundation front included) is reported, not held
-    to 5 cm: on 200 scenarios it is about 7.4 cm for both networks."""
+    to 5 cm: on 200 scenarios it is about 7.4 cm for both networks.  The
+    learned eta_max readout of the configured network's run must lower the
+    zone peak error and keep the coastal peak and runup bars; with the
+    residual fallback it holds the zone peak to 5 cm at >= 8x (4.7 cm at
+    9.4x when written)."""
     from tsuner_amd import correction as K
+    from tsuner_amd import eta_readout
     nets = {"conv4_w32": K.load_net(K.WEIGHTS_4X32), "conv6_w64": K.load_net(K.WEIGHTS)}
     net, so, _ = nets["conv4_w32"]
     ev = K.evaluate(net, so, list(range(300001, 300201)), fine_steps=3000,
-                    others={"conv6_w64": nets["conv6_w64"][:2]})
+                    others={"conv6_w64": nets["conv6_w64"][:2]}, readout=eta_readout.load())
     base = ev["coarse_alone"]
     rows = {"conv4_w32": ev["coarse_nn"], "conv6_w64": ev["others"]["conv6_w64"]["coarse_nn"]}
     print("coarse_alone", base, flush=True)
@@ -83,3 +88,17 @@ def test_networks_meet_the_bars_and_beat_coarse_alone():
         assert corr["peak_eta_err_mean_m"] < base["peak_eta_err_mean_m"]
         assert corr["nearshore_mean_abs_eta_err_m"] < 0.8 * base["nearshore_mean_abs_eta_err_m"]
         assert corr["runup_rel_err_mean"] < base["runup_rel_err_mean"]
+    ro = ev["coarse_nn_readout"]
+    print("conv4_w32 + eta readout", ro, flush=True)
+    for name, pol in ev["readout_fallback_policies"]["policies"].items():
+        print("readout fallback", name, {k: pol[k] for k in ("peak_eta_err_mean_m", "fallback_frac",
+                                                             "speedup_vs_fine")}, flush=True)
+    assert ro["peak_eta_err_mean_m"] < 0.85 * ev["coarse_nn"]["peak_eta_err_mean_m"]
+    assert ro["peak_eta_sea_err_mean_m"] <= 0.05 and ro["runup_rel_err_mean"] <= 0.10
+    assert ro["speedup_vs_fine"] >= 8.0
+    # with the residual fallback (threshold from scenarios 300001-300100,
+    # reported on 300101-300200): the zone peak bar at >= 8x
+    pol = ev["readout_fallback_policies"]["policies"]["resid_max"]
+    print("readout + resid_max fallback", pol, flush=True)
+    assert pol["peak_eta_err_mean_m"] <= 0.05 and pol["speedup_vs_fine"] >= 8.0
+    assert pol["peak_eta_sea_err_mean_m"] <= 0.05 and pol["runup_rel_err_mean"] <= 0.10

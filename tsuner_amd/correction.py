@@ -488,12 +488,14 @@ def load_net(path: str = WEIGHTS, max_batch: int = 32, h: int = None, w: int = N
 
 
 def evaluate(net, scale_out, seeds: Sequence[int], fine_steps: int = 3000, resid_tau: float = 0.5, fine_n: int = FINE_N,
-             max_batch: int = 32, min_speedup: float = 8.0, others: dict = None) -> dict:
+             max_batch: int = 32, min_speedup: float = 8.0, others: dict = None, readout: dict = None) -> dict:
     """Held-out evaluation: fine members (fine_steps steps) against the coarse
     solver alone and coarse + network, with the residual fallback.  others
     ({name: (net, scale_out)}) are further networks evaluated against the
     same fine and coarse-alone runs (their summaries and fallback policies
-    under out["others"][name])."""
+    under out["others"][name]).  readout (an eta_readout model for `net`):
+    the coarse + network run's eta_max read out by it, reported as
+    out["coarse_nn_readout"] with its own fallback policies."""
     t0 = time.perf_counter()
     fine_sc = [fine_scenario(s, extrema=1, n=fine_n) for s in seeds]
     coarse_sc = [coarse_scenario(f) for f in fine_sc]
@@ -580,6 +582,30 @@ def evaluate(net, scale_out, seeds: Sequence[int], fine_steps: int = 3000, resid
                            for tau in (0.05, 0.1, 0.2, 0.3, 0.5)],
     }
     out["fallback_policies"] = _fallback_policies(mb, mc, corr, base, fine_per, min_speedup, summary)
+    if readout is not None:
+        # the readout's eta_max field for the eta_max metrics (zone peak,
+        # coastal peak, nearshore mean); runup stays the run's own reading
+        from . import eta_readout
+        import dataclasses
+        t_ro = time.perf_counter()
+        ro = dataclasses.replace(corr, eta_max=[eta_readout.apply(readout, corr.eta_max[i], corr.hmax[i], corr.H0[i],
+                                                                  fine.H0[i]) for i in range(n)])
+        t_ro = time.perf_counter() - t_ro
+        mr = []
+        for i in range(n):
+            m = metrics(fine, ro, i)
+            for k in ("runup_fine", "runup_coarse", "runup_rel_err", "runup_coarse_ground",
+                      "runup_rel_err_coarse_ground"):
+                m[k] = mc[i][k]
+            mr.append(m)
+        wall_ro = corr.wall_s + t_ro
+        out["coarse_nn_readout"] = dict(summary(mr), wall_s=wall_ro, readout_s=t_ro, sims_per_hr=n * 3600 / wall_ro,
+                                        speedup_vs_fine=fine.wall_s / wall_ro)
+        # fallback costs with the readout's own time in the coarse wall
+        out["readout_fallback_policies"] = _fallback_policies(mb, mr, dataclasses.replace(corr, wall_s=wall_ro), base,
+                                                              fine_per, min_speedup, summary)
+        log("[eval] readout: zone peak %.4f m (network alone %.4f m)"
+            % (out["coarse_nn_readout"]["peak_eta_err_mean_m"], out["coarse_nn"]["peak_eta_err_mean_m"]))
     disagree = np.array([abs(mc[i]["peak_eta_coarse"] - mb[i]["peak_eta_coarse"]) for i in range(n)])
     if others:
         out["others"] = {}
