@@ -584,7 +584,9 @@ def evaluate(net, scale_out, seeds: Sequence[int], fine_steps: int = 3000, resid
     out["fallback_policies"] = _fallback_policies(mb, mc, corr, base, fine_per, min_speedup, summary)
     if readout is not None:
         # the readout's eta_max field for the eta_max metrics (zone peak,
-        # coastal peak, nearshore mean); runup stays the run's own reading
+        # coastal peak, nearshore mean); runup stays the run's own reading.
+        # fine.H0 is the fine scenarios' input bathymetry (copied before the
+        # fine run), which a deployment has without running the fine grid.
         from . import eta_readout
         import dataclasses
         t_ro = time.perf_counter()
