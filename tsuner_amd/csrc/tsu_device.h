@@ -834,11 +834,13 @@ __device__ __forceinline__ void riemann_pair(const DevParams& P, const bool (&mo
         rp_speeds<R, false>(e[1], sp[1]);
     }
     for (int i = 0; i < 2; i++) rp_solve<R, SED, FB>(P, mobile[i], e[i], sp[i], Dm[i], Dp[i]);
-    for (int i = 0; i < 2; i++)
-        for (int f = 0; f < NF; f++) {
-            Dm[i][f] = dry[i] ? (R)0 : Dm[i][f];
-            Dp[i][f] = dry[i] ? (R)0 : Dp[i][f];
-        }
+    // dry lanes are on the coast: away from it no lane of the wave has one
+    if (wave_any(dry[0] || dry[1]))
+        for (int i = 0; i < 2; i++)
+            for (int f = 0; f < NF; f++) {
+                Dm[i][f] = dry[i] ? (R)0 : Dm[i][f];
+                Dp[i][f] = dry[i] ? (R)0 : Dp[i][f];
+            }
 }
 
 // Reconstructor.cu:57-88 with SedFlux.cuh:51-99: exchange rate phi between
