@@ -38,9 +38,9 @@ HBM_PEAK_GBS = 8000.0
 # TRAFFIC_PROFILE.  None at other sizes.  The stage kernels' work depends on
 # the state (waves at rest skip quotients and Riemann solves), their HBM
 # traffic does not.
-TRAFFIC_8192 = {"fp64": 7.197e9, "fp32": 5.828e9}
+TRAFFIC_8192 = {"fp64": 7.203e9, "fp32": 5.810e9}
 TRAFFIC_PROFILE = "profiles/stage_kernel_8192_counters.md"
-PROFILE_BUILD = "2fd4814"
+PROFILE_BUILD = "593c74d"
 METRIC = "cell-updates/s at 8192² FP64 vs FP32; ensemble sims/hr coarse+NN vs fine"
 
 
