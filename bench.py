@@ -340,10 +340,13 @@ def run_correction(heldout, fine_steps):
     names = list(nets)
     head = names[0]
     net, scale_out, info = nets[head]
-    others = {k: (nets[k][0], nets[k][1]) for k in names[1:]}
-    # the eta_max readout trained on the headline network's runs
+    # each network's eta_max readout, trained on that network's runs
     from tsuner_amd import eta_readout
-    readout = eta_readout.load() if head == "conv4_w32" and os.path.exists(eta_readout.WEIGHTS) else None
+    ro_paths = {"conv4_w32": eta_readout.WEIGHTS, "conv6_w64": eta_readout.WEIGHTS_6X64}
+    readouts = {k: (eta_readout.load(ro_paths[k]) if k in ro_paths and os.path.exists(ro_paths[k]) else None)
+                for k in names}
+    others = {k: (nets[k][0], nets[k][1], readouts[k]) for k in names[1:]}
+    readout = readouts[head]
     ev = K.evaluate(net, scale_out, list(range(300001, 300001 + heldout)), fine_steps=fine_steps, others=others,
                     readout=readout)
     rows = {head: {"coarse_nn": pick(ev["coarse_nn"]),
@@ -358,6 +361,10 @@ def run_correction(heldout, fine_steps):
         rows[k] = {"coarse_nn": pick(o["coarse_nn"]),
                    "fallback_policies": {p: pick(v) for p, v in o["fallback_policies"]["policies"].items()},
                    "network": network(nets[k][0], nets[k][2])}
+        if "coarse_nn_readout" in o:
+            rows[k]["coarse_nn_eta_readout"] = pick(o["coarse_nn_readout"])
+            rows[k]["eta_readout_fallback_policies"] = {
+                p: pick(v) for p, v in o["readout_fallback_policies"]["policies"].items()}
     return {"workload": "%d held-out Gaussian-source scenarios, fine 1024^2 dx=10 m (%d steps) vs coarse 256^2 "
                         "dx=40 m to the same end times, FP32; network correction every %d coarse steps"
                         % (heldout, fine_steps, K.WINDOW),

@@ -75,7 +75,8 @@ def test_networks_meet_the_bars_and_beat_coarse_alone():
     nets = {"conv4_w32": K.load_net(K.WEIGHTS_4X32), "conv6_w64": K.load_net(K.WEIGHTS)}
     net, so, _ = nets["conv4_w32"]
     ev = K.evaluate(net, so, list(range(300001, 300201)), fine_steps=3000,
-                    others={"conv6_w64": nets["conv6_w64"][:2]}, readout=eta_readout.load())
+                    others={"conv6_w64": nets["conv6_w64"][:2] + (eta_readout.load(eta_readout.WEIGHTS_6X64),)},
+                    readout=eta_readout.load())
     base = ev["coarse_alone"]
     rows = {"conv4_w32": ev["coarse_nn"], "conv6_w64": ev["others"]["conv6_w64"]["coarse_nn"]}
     print("coarse_alone", base, flush=True)
@@ -98,6 +99,13 @@ def test_networks_meet_the_bars_and_beat_coarse_alone():
     assert ro["speedup_vs_fine"] >= 8.0
     # with the residual fallback (threshold from scenarios 300001-300100,
     # reported on 300101-300200): the zone peak bar at >= 8x
+    ro64 = ev["others"]["conv6_w64"]["coarse_nn_readout"]
+    print("conv6_w64 + eta readout", ro64, flush=True)
+    for name, pol in ev["others"]["conv6_w64"]["readout_fallback_policies"]["policies"].items():
+        print("conv6_w64 readout fallback", name, {k: pol[k] for k in ("peak_eta_err_mean_m", "fallback_frac",
+                                                                       "speedup_vs_fine")}, flush=True)
+    assert ro64["peak_eta_err_mean_m"] < 0.85 * ev["others"]["conv6_w64"]["coarse_nn"]["peak_eta_err_mean_m"]
+    assert ro64["peak_eta_sea_err_mean_m"] <= 0.05 and ro64["runup_rel_err_mean"] <= 0.10
     pol = ev["readout_fallback_policies"]["policies"]["resid_max"]
     print("readout + resid_max fallback", pol, flush=True)
     assert pol["peak_eta_err_mean_m"] <= 0.05 and pol["speedup_vs_fine"] >= 8.0

@@ -493,8 +493,9 @@ def evaluate(net, scale_out, seeds: Sequence[int], fine_steps: int = 3000, resid
     solver alone and coarse + network, with the residual fallback.  others
     ({name: (net, scale_out)}) are further networks evaluated against the
     same fine and coarse-alone runs (their summaries and fallback policies
-    under out["others"][name]).  readout (an eta_readout model for `net`):
-    the coarse + network run's eta_max read out by it, reported as
+    under out["others"][name]; a third tuple element is that network's
+    eta_max readout).  readout (an eta_readout model for `net`): the coarse
+    + network run's eta_max read out by it, reported as
     out["coarse_nn_readout"] with its own fallback policies."""
     t0 = time.perf_counter()
     fine_sc = [fine_scenario(s, extrema=1, n=fine_n) for s in seeds]
@@ -582,36 +583,43 @@ def evaluate(net, scale_out, seeds: Sequence[int], fine_steps: int = 3000, resid
                            for tau in (0.05, 0.1, 0.2, 0.3, 0.5)],
     }
     out["fallback_policies"] = _fallback_policies(mb, mc, corr, base, fine_per, min_speedup, summary)
-    if readout is not None:
-        # the readout's eta_max field for the eta_max metrics (zone peak,
-        # coastal peak, nearshore mean); runup stays the run's own reading.
-        # fine.H0 is the fine scenarios' input bathymetry (copied before the
-        # fine run), which a deployment has without running the fine grid.
+
+    def read_out(run, m_run, model):
+        """The run's eta_max read out by `model` for the eta_max metrics
+        (zone peak, coastal peak, nearshore mean); runup stays the run's own
+        reading.  fine.H0 is the fine scenarios' input bathymetry (copied
+        before the fine run), which a deployment has without running the
+        fine grid.  Returns (summary, fallback policies), the readout's own
+        time included in the coarse wall."""
         from . import eta_readout
         import dataclasses
         t_ro = time.perf_counter()
-        ro = dataclasses.replace(corr, eta_max=[eta_readout.apply(readout, corr.eta_max[i], corr.hmax[i], corr.H0[i],
-                                                                  fine.H0[i]) for i in range(n)])
+        ro = dataclasses.replace(run, eta_max=[eta_readout.apply(model, run.eta_max[i], run.hmax[i], run.H0[i],
+                                                                 fine.H0[i]) for i in range(n)])
         t_ro = time.perf_counter() - t_ro
         mr = []
         for i in range(n):
             m = metrics(fine, ro, i)
             for k in ("runup_fine", "runup_coarse", "runup_rel_err", "runup_coarse_ground",
                       "runup_rel_err_coarse_ground"):
-                m[k] = mc[i][k]
+                m[k] = m_run[i][k]
             mr.append(m)
-        wall_ro = corr.wall_s + t_ro
-        out["coarse_nn_readout"] = dict(summary(mr), wall_s=wall_ro, readout_s=t_ro, sims_per_hr=n * 3600 / wall_ro,
-                                        speedup_vs_fine=fine.wall_s / wall_ro)
-        # fallback costs with the readout's own time in the coarse wall
-        out["readout_fallback_policies"] = _fallback_policies(mb, mr, dataclasses.replace(corr, wall_s=wall_ro), base,
-                                                              fine_per, min_speedup, summary)
+        wall_ro = run.wall_s + t_ro
+        summ = dict(summary(mr), wall_s=wall_ro, readout_s=t_ro, sims_per_hr=n * 3600 / wall_ro,
+                    speedup_vs_fine=fine.wall_s / wall_ro)
+        pol = _fallback_policies(mb, mr, dataclasses.replace(run, wall_s=wall_ro), base, fine_per, min_speedup,
+                                 summary)
+        return summ, pol
+
+    if readout is not None:
+        out["coarse_nn_readout"], out["readout_fallback_policies"] = read_out(corr, mc, readout)
         log("[eval] readout: zone peak %.4f m (network alone %.4f m)"
             % (out["coarse_nn_readout"]["peak_eta_err_mean_m"], out["coarse_nn"]["peak_eta_err_mean_m"]))
     disagree = np.array([abs(mc[i]["peak_eta_coarse"] - mb[i]["peak_eta_coarse"]) for i in range(n)])
     if others:
         out["others"] = {}
-        for name, (net2, so2) in others.items():
+        for name, entry in others.items():
+            net2, so2 = entry[0], entry[1]
             c2 = run_coarse(coarse_sc, fine.t_end, net=net2, scale_out=so2, shore_h=SHORE_H)
             log("[eval] coarse + network %s: %.2f s wall" % (name, c2.wall_s))
             m2 = [metrics(fine, c2, i) for i in range(n)]
@@ -622,6 +630,11 @@ def evaluate(net, scale_out, seeds: Sequence[int], fine_steps: int = 3000, resid
                 "per_scenario": {"peak_eta_err_nn": [m["peak_eta_err"] for m in m2],
                                  "peak_eta_nn": [m["peak_eta_coarse"] for m in m2],
                                  "runup_rel_err_nn": [m["runup_rel_err"] for m in m2]}}
+            if len(entry) > 2 and entry[2] is not None:
+                o = out["others"][name]
+                o["coarse_nn_readout"], o["readout_fallback_policies"] = read_out(c2, m2, entry[2])
+                log("[eval] readout %s: zone peak %.4f m (network alone %.4f m)"
+                    % (name, o["coarse_nn_readout"]["peak_eta_err_mean_m"], o["coarse_nn"]["peak_eta_err_mean_m"]))
     out["per_scenario"] = {"seed": [int(s) for s in seeds],
                            "peak_eta_fine": [m["peak_eta_fine"] for m in mc],
                            "peak_eta_err_coarse": [m["peak_eta_err"] for m in mb],

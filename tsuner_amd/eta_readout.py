@@ -41,7 +41,8 @@ from . import correction as K
 BAND = 1.0                 # m below the coarse zone peak
 MAX_CELLS = 3000           # at most this many cells per member, the highest
 NEARSHORE_DEPTH = 10.0     # zone: coarse cells initially shallower than this [m]
-WEIGHTS = os.path.join(K.HERE, "eta_readout_4x32.npz")
+WEIGHTS = os.path.join(K.HERE, "eta_readout_4x32.npz")          # for the 4x32 network's runs
+WEIGHTS_6X64 = os.path.join(K.HERE, "eta_readout_6x64.npz")     # for the 64-wide network's runs
 NFEAT = 15
 
 
@@ -229,14 +230,17 @@ def main(argv=None):
         if a.samples:
             np.savez_compressed(a.samples, X=X, y=y, g=g, peak=peak, seeds=a.seeds,
                                 network=os.path.basename(a.network))
+    network = os.path.basename(a.network)
     if a.cmd == "fit":
         z = np.load(a.samples)
         X, y, g, peak = z["X"], z["y"], z["g"], z["peak"]
+        if "network" in z:
+            network = str(z["network"])
     if a.cmd in ("fit", "train"):
         model = fit(X, y, g, peak, epochs=a.epochs)
         info = {"train_seeds": "1-%d" % int(g.max() + 1), "samples": int(len(X)),
                 "wet_samples": int(np.isfinite(y).sum()), "epochs": a.epochs, "band_m": BAND,
-                "fold_bias_m": float(model["fold_bias"]), "network": os.path.basename(a.network)}
+                "fold_bias_m": float(model["fold_bias"]), "network": network}
         save(a.out, model, info)
         K.log("[readout] saved %s: %s" % (a.out, info))
 
