@@ -127,12 +127,19 @@ def test_fit_recovers_a_constant_offset():
     assert err.mean() < 0.01 and err.max() < 0.025, err
 
 
-def test_shipped_weights():
-    if not os.path.exists(E.WEIGHTS):
-        pytest.skip("no shipped readout weights")
-    m = E.load()
+@pytest.mark.parametrize("path", [E.WEIGHTS, E.WEIGHTS_6X64])
+def test_shipped_weights(path):
+    if not os.path.exists(path):
+        pytest.skip("no shipped readout weights at %s" % path)
+    m = E.load(path)
     assert m["W1"].shape == (E.NFEAT, 64) and m["W2"].shape == (64, 64) and m["W3"].shape == (64, 1)
     assert np.all(m["sd"] > 0)
     eta, hmax, H0c, H0f = _case(seed=5)
     out = E.apply(m, eta, hmax, H0c, H0f)
     assert np.all(np.isfinite(out[np.isfinite(eta)]))
+
+
+def test_silu_saturates_without_warnings():
+    with np.errstate(all="raise"):
+        y = E._silu(np.array([-1000.0, -50.0, 0.0, 50.0]))
+    assert y[0] == 0 and abs(y[1]) < 1e-18 and y[2] == 0 and y[3] == 50.0

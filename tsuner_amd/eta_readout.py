@@ -96,7 +96,10 @@ def features(eta_max: np.ndarray, hmax: np.ndarray, H0c: np.ndarray, H0f: np.nda
 
 
 def _silu(x):
-    return x / (1.0 + np.exp(-x))
+    # exp(-x) overflows to inf for x below about -709; x / inf is the -0
+    # the function tends to there
+    with np.errstate(over="ignore"):
+        return x / (1.0 + np.exp(-x))
 
 
 def predict(model: dict, X: np.ndarray) -> np.ndarray:
