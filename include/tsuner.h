@@ -333,6 +333,12 @@ TSU_API const char *tsu_version(void);
 TSU_API int tsu_kat_edge_flux(const tsu_sim_params *params, const tsu_scheme *scheme, int n,
                               const double *left, const double *right, const int *normal,
                               const int *fixed, double *Dm, double *Dp, int device);
+/* The same with edges 2k and 2k+1 solved together, as the stage kernels
+ * solve a row's x and y edges (one pass through the solver's phases with
+ * wave-wide tests over both); n even, otherwise TSU_ERR_ARG. */
+TSU_API int tsu_kat_edge_flux_pairs(const tsu_sim_params *params, const tsu_scheme *scheme, int n,
+                                    const double *left, const double *right, const int *normal,
+                                    const int *fixed, double *Dm, double *Dp, int device);
 /* what: 0 limiter(a,b) -> o1; 1 velocity(h=a,qx=b,qy=c) -> o1,o2,o3=|u|;
  * 2 state filter(a,b,c,d) -> o1..o4; 3 cell dt(a,b,c) -> o1;
  * 4 erosion rate(h=a,qx=b,qy=c,hc=d) -> o1; 5 bedload(h=a,un=b,ut=c) -> o1;

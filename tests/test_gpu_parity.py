@@ -156,6 +156,91 @@ def test_kat_edge_flux_waves_at_rest(oracle):
     assert moving >= 4     # the single moving lanes do produce fluxes
 
 
+def _kat_pairs(p, s, L, R, normals, fixed):
+    n = len(L)
+    Dm = np.zeros((n, 5))
+    Dp = np.zeros((n, 5))
+    lib = _lib.lib()
+    _lib.check(lib.tsu_kat_edge_flux_pairs(_lib.make_params(p), _lib.make_scheme(s), n,
+                                           _lib.ptr(np.ascontiguousarray(L)), _lib.ptr(np.ascontiguousarray(R)),
+                                           _lib.ptr(np.ascontiguousarray(normals), _lib.C.c_int),
+                                           _lib.ptr(np.ascontiguousarray(fixed), _lib.C.c_int), _lib.ptr(Dm),
+                                           _lib.ptr(Dp), 0), "kat pairs")
+    return Dm, Dp
+
+
+def _check_edges(oracle, p, s, L, R, normals, fixed, Dm, Dp, what):
+    for i in range(len(L)):
+        dm, dp = oracle.edge_flux(p, s, L[i], R[i], normals[i], fixed[i])
+        if not (np.array_equal(dm, Dm[i]) and np.array_equal(dp, Dp[i])):
+            raise AssertionError("%s, edge %d (pair %d): L=%r R=%r n=%r fixed=%d\n oracle Dm=%r Dp=%r\n"
+                                 " gpu Dm=%r Dp=%r" % (what, i, i // 2, L[i], R[i], normals[i], fixed[i], dm, dp,
+                                                       Dm[i], Dp[i]))
+
+
+@pytest.mark.parametrize("law, tiny_shields", [(S.BEDLOAD_AM, False), (S.BEDLOAD_GRASS, False),
+                                               (S.BEDLOAD_AM, True)])
+def test_kat_edge_flux_pairs(oracle, law, tiny_shields):
+    """Random wet/dry edges solved two at a time by one lane (the stage
+    kernels' paired solve: one pass through the solver's phases with the
+    wave-wide tests taken over both edges), each edge bitwise against the
+    oracle."""
+    rng = np.random.default_rng(31 + law + 5 * tiny_shields)
+    n = 3000
+    L, R = random_faces(rng, n), random_faces(rng, n)
+    L[:500, 0] = rng.uniform(0, 3e-3, 500)
+    R[:500, 0] = rng.uniform(0, 3e-3, 500)
+    normals = np.array([(1, 0), (-1, 0), (0, 1), (0, -1)], dtype=np.int32)[rng.integers(0, 4, n)]
+    fixed = rng.integers(0, 2, n).astype(np.int32)
+    sc = S.sediment_shelf(n=8, dx=10.0, inflow=False, bedload_law=law)
+    p, s = sc.params, sc.scheme
+    if tiny_shields:
+        p = dict(p, OC=1e-220)
+    Dm, Dp = _kat_pairs(p, s, L, R, normals, fixed)
+    _check_edges(oracle, p, s, L, R, normals, fixed, Dm, Dp, "random pairs")
+
+
+def test_kat_edge_pairs_that_split_the_votes(oracle):
+    """Pairs whose two edges take different sides of a wave-wide test, in
+    blocks of 64 pairs (one wave): an edge dry on both sides next to a
+    moving one; an edge at rest next to a moving one; an edge whose depth
+    needs the scaled square root (1e-240 m) next to ordinary ones; and whole
+    waves at rest.  Each edge bitwise against the oracle."""
+    rng = np.random.default_rng(5)
+    blocks = []
+    for case in range(4):
+        m = 128                                   # 64 pairs: edges 2k (a) and 2k+1 (b)
+        h = rng.uniform(0.5, 20.0, m)
+        H = rng.uniform(-5.0, 15.0, m)
+        L = np.stack([h, 0 * h, 0 * h, H, 0 * h], axis=1)
+        R = L.copy()
+        moving = random_faces(rng, m)
+        moving[:, 0] = rng.uniform(0.1, 10.0, m)
+        if case == 0:                             # a dry on both sides, b moving
+            L[0::2, 0] = 0.0
+            R[0::2, 0] = 0.0
+            L[1::2], R[1::2] = moving[1::2], random_faces(rng, m)[1::2]
+        elif case == 1:                           # a at rest, b moving
+            L[1::2], R[1::2] = moving[1::2], random_faces(rng, m)[1::2]
+        elif case == 2:                           # a needs the scaled square root in some lanes
+            L[:], R[:] = moving, random_faces(rng, m)
+            L[0:64:8, 0] = 1e-240
+            R[0:64:8, 0] = 3e-241
+        # case 3: every edge at rest
+        blocks.append((L, R))
+    L = np.concatenate([b[0] for b in blocks])
+    R = np.concatenate([b[1] for b in blocks])
+    n = len(L)
+    normals = np.tile(np.array([(1, 0), (0, 1), (-1, 0), (0, 1)], dtype=np.int32), (n // 4, 1))
+    fixed = np.ones(n, dtype=np.int32)
+    sc = S.sediment_shelf(n=8, dx=10.0, inflow=False)
+    p, s = sc.params, sc.scheme
+    Dm, Dp = _kat_pairs(p, s, L, R, normals, fixed)
+    _check_edges(oracle, p, s, L, R, normals, fixed, Dm, Dp, "split votes")
+    assert np.any(Dm[1:256:2] != 0)              # the moving edges do produce fluxes
+    assert not np.any(Dm[384:]) and not np.any(Dp[384:])
+
+
 def kat_point(p, s, what, a, b=None, c=None, d=None):
     n = len(a)
     arr = [np.ascontiguousarray(x, dtype=np.float64) if x is not None else None for x in (a, b, c, d)]

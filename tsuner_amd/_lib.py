@@ -147,6 +147,8 @@ _PROTOS = {
     "tsu_version": (C.c_char_p, []),
     "tsu_kat_edge_flux": (C.c_int, [C.POINTER(SimParams), C.POINTER(Scheme), C.c_int, _dp, _dp, _ip, _ip, _dp,
                                     _dp, C.c_int]),
+    "tsu_kat_edge_flux_pairs": (C.c_int, [C.POINTER(SimParams), C.POINTER(Scheme), C.c_int, _dp, _dp, _ip, _ip,
+                                          _dp, _dp, C.c_int]),
     "tsu_kat_point": (C.c_int, [C.POINTER(SimParams), C.POINTER(Scheme), C.c_int, C.c_int, _dp, _dp, _dp, _dp,
                                 _dp, _dp, _dp, _dp, C.c_int]),
     "tsu_write_output": (C.c_int, [C.c_char_p, C.POINTER(SimParams), C.c_int, _dp, _dp, _dp, _dp, _dp]),

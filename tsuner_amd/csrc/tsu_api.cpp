@@ -753,12 +753,17 @@ TSU_API int tsu_bandwidth_probe(int device, long long bytes, int reps, double* g
     return TSU_OK;
 }
 
-TSU_API int tsu_kat_edge_flux(const tsu_sim_params* p, const tsu_scheme* s, int n, const double* left,
-                              const double* right, const int* normal, const int* fixed, double* Dm, double* Dp,
-                              int device)
+namespace {
+
+// Shared host side of the edge known-answer entries: pairs = false solves
+// every edge alone (edge_flux), true solves edges 2k and 2k + 1 together
+// (edge_flux_pair, n even).
+int kat_edges(const tsu_sim_params* p, const tsu_scheme* s, int n, const double* left, const double* right,
+              const int* normal, const int* fixed, double* Dm, double* Dp, int device, bool pairs)
 {
     if (!p || !s || n < 0 || (n > 0 && (!left || !right || !normal || !fixed || !Dm || !Dp)))
         return fail(TSU_ERR_ARG, "bad argument");
+    if (pairs && n % 2 != 0) return fail(TSU_ERR_ARG, "edge pairs need an even number of edges");
     if (n == 0) return TSU_OK;
     DevParams P;
     tsu::make_dev_params(p, s, &P);
@@ -785,7 +790,9 @@ TSU_API int tsu_kat_edge_flux(const tsu_sim_params* p, const tsu_scheme* s, int 
     if (e == hipSuccess) e = hipMemcpy(dr, right, b5, hipMemcpyHostToDevice);
     if (e == hipSuccess) e = hipMemcpy(dk, kind.data(), bi, hipMemcpyHostToDevice);
     if (e == hipSuccess) e = hipMemcpy(df, fixed, bi, hipMemcpyHostToDevice);
-    if (e == hipSuccess) e = tsu::launch_kat_edge(P, n, dl, dr, dk, df, dm, dp, nullptr);
+    if (e == hipSuccess)
+        e = pairs ? tsu::launch_kat_edge_pairs(P, n / 2, dl, dr, dk, df, dm, dp, nullptr)
+                  : tsu::launch_kat_edge(P, n, dl, dr, dk, df, dm, dp, nullptr);
     if (e == hipSuccess) e = hipMemcpy(Dm, dm, b5, hipMemcpyDeviceToHost);
     if (e == hipSuccess) e = hipMemcpy(Dp, dp, b5, hipMemcpyDeviceToHost);
     (void)hipFree(dl);
@@ -796,6 +803,22 @@ TSU_API int tsu_kat_edge_flux(const tsu_sim_params* p, const tsu_scheme* s, int 
     (void)hipFree(df);
     if (e != hipSuccess) return fail(TSU_ERR_HIP, std::string("kat edge: ") + hipGetErrorString(e));
     return TSU_OK;
+}
+
+}  // namespace
+
+TSU_API int tsu_kat_edge_flux(const tsu_sim_params* p, const tsu_scheme* s, int n, const double* left,
+                              const double* right, const int* normal, const int* fixed, double* Dm, double* Dp,
+                              int device)
+{
+    return kat_edges(p, s, n, left, right, normal, fixed, Dm, Dp, device, false);
+}
+
+TSU_API int tsu_kat_edge_flux_pairs(const tsu_sim_params* p, const tsu_scheme* s, int n, const double* left,
+                                    const double* right, const int* normal, const int* fixed, double* Dm,
+                                    double* Dp, int device)
+{
+    return kat_edges(p, s, n, left, right, normal, fixed, Dm, Dp, device, true);
 }
 
 TSU_API int tsu_kat_point(const tsu_sim_params* p, const tsu_scheme* s, int what, int n, const double* a,

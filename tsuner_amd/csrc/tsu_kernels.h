@@ -73,6 +73,10 @@ int mass_blocks(long long nvol);
 // Known-answer kernels running the device functions on arrays of inputs.
 hipError_t launch_kat_edge(const DevParams& P, int n, const double* left, const double* right, const int* kind,
                            const int* fixed, double* Dm, double* Dp, hipStream_t s);
+// Edges 2k and 2k + 1 of the same arrays solved as a pair (the stage
+// kernels' edge_flux_pair), npairs pairs.
+hipError_t launch_kat_edge_pairs(const DevParams& P, int npairs, const double* left, const double* right,
+                                 const int* kind, const int* fixed, double* Dm, double* Dp, hipStream_t s);
 hipError_t launch_kat_point(int what, int law, int n, const double* a, const double* b, const double* c,
                             const double* d, double* o1, double* o2, double* o3, double* o4, const DevParams& P,
                             hipStream_t s);

@@ -957,6 +957,32 @@ __global__ void kat_edge_kernel(DevParams P, int n, const double* left, const do
     }
 }
 
+// Edges 2k and 2k + 1 solved together by lane k, as the stage kernels
+// solve a row's x edge and y edge (edge_flux_pair).
+template <bool FB>
+__global__ void kat_edge_pair_kernel(DevParams P, int npairs, const double* left, const double* right,
+                                     const int* kind, const int* fixed, double* Dm, double* Dp)
+{
+    const int k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= npairs) return;
+    const int a = 2 * k, b = 2 * k + 1;
+    double La[5], Ra[5], Lb[5], Rb[5], dma[5], dpa[5], dmb[5], dpb[5];
+    for (int f = 0; f < 5; f++) {
+        La[f] = left[5 * a + f];
+        Ra[f] = right[5 * a + f];
+        Lb[f] = left[5 * b + f];
+        Rb[f] = right[5 * b + f];
+    }
+    edge_flux_pair<double, true, FB>(P, kind[a], fixed[a] != 0 && P.bedload, La, Ra, dma, dpa, kind[b],
+                                     fixed[b] != 0 && P.bedload, Lb, Rb, dmb, dpb);
+    for (int f = 0; f < 5; f++) {
+        Dm[5 * a + f] = dma[f];
+        Dp[5 * a + f] = dpa[f];
+        Dm[5 * b + f] = dmb[f];
+        Dp[5 * b + f] = dpb[f];
+    }
+}
+
 // what: 0 limiter(a, b); 1 velocity(a=h, b=qx, c=qy); 2 state filter
 // (a, b, c, d) = (h, qx, qy, hc); 3 cell dt (a, b, c); 4 erosion rate
 // (h, qx, qy, hc); 5 bedload (h, un, ut).
@@ -1144,6 +1170,18 @@ hipError_t launch_kat_edge(const DevParams& P, int n, const double* left, const 
     else
         hipLaunchKernelGGL(kat_edge_kernel<false>, dim3((n + 63) / 64), dim3(64), 0, s, P, n, left, right, kind,
                            fixed, Dm, Dp);
+    return hipGetLastError();
+}
+
+hipError_t launch_kat_edge_pairs(const DevParams& P, int npairs, const double* left, const double* right,
+                                 const int* kind, const int* fixed, double* Dm, double* Dp, hipStream_t s)
+{
+    if (fast_bedload(P))
+        hipLaunchKernelGGL(kat_edge_pair_kernel<true>, dim3((npairs + 63) / 64), dim3(64), 0, s, P, npairs, left,
+                           right, kind, fixed, Dm, Dp);
+    else
+        hipLaunchKernelGGL(kat_edge_pair_kernel<false>, dim3((npairs + 63) / 64), dim3(64), 0, s, P, npairs, left,
+                           right, kind, fixed, Dm, Dp);
     return hipGetLastError();
 }
 
