@@ -118,3 +118,29 @@ def test_network_kernel_kinds_validated_before_device_use():
         assert L.tsu_nn_wgrad_workspace(kind) == 9 * 256 * my * kx + 256 * my, kind
     for kind in (0, 4, 8, 6 | 32):
         assert L.tsu_nn_wgrad_workspace(kind) == -1, kind
+
+
+def test_edge_kat_arguments_rejected_before_device_use():
+    """The edge known-answer entries check their arguments on the host:
+    null buffers, a negative count, an odd count for the paired entry and a
+    normal that is not an axis direction return TSU_ERR_ARG (3) with a
+    message, and no device is touched."""
+    from tsuner_amd import scenarios as S
+    sc = S.sediment_shelf(n=8, dx=10.0, inflow=False)
+    p, s = _lib.make_params(sc.params), _lib.make_scheme(sc.scheme)
+    L = _lib.lib()
+    faces = np.zeros((3, 5))
+    normals = np.array([(1, 0), (0, 1), (0, -1)], dtype=np.int32)
+    fixed = np.zeros(3, dtype=np.int32)
+    out = np.zeros((3, 5))
+    args = lambda n, nrm=normals: (p, s, n, _lib.ptr(faces), _lib.ptr(faces), _lib.ptr(nrm, C.c_int),
+                                   _lib.ptr(fixed, C.c_int), _lib.ptr(out), _lib.ptr(out), 0)
+    assert L.tsu_kat_edge_flux_pairs(*args(3)) == 3
+    assert b"even" in L.tsu_last_error()
+    for entry in (L.tsu_kat_edge_flux, L.tsu_kat_edge_flux_pairs):
+        assert entry(*args(-2)) == 3
+        assert entry(p, s, 2, None, None, None, None, None, None, 0) == 3
+        bad = np.array([(1, 1), (0, 1)], dtype=np.int32)
+        assert entry(*args(2, bad)) == 3
+        assert b"axis-aligned" in L.tsu_last_error()
+        assert entry(*args(0)) == 0
