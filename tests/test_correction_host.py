@@ -120,3 +120,33 @@ def test_subgrid_runup_reads_the_fine_ground():
     # a coarse cell never reached floods nothing
     assert K.runup_subgrid(H0, np.full((n, n), np.nan), np.zeros((n, n))) == 0.0
 
+
+
+def test_readout_metrics_move_only_the_front_cells():
+    """readout_metrics with a readout that adds 5 cm to every cell it reads
+    out: the zone peak, which sits on wetted land, rises by 5 cm; the sea
+    cells' peak and every runup entry stay those of the run itself."""
+    from tsuner_amd import eta_readout as E
+    n = 16
+    H0f = np.tile(np.linspace(20.0, -4.0, 4 * n), (4 * n, 1))
+    wet_f = H0f > -2.0
+    ef = np.where(wet_f, np.where(H0f > 0, 0.5, 0.5 + 0.01 * (-H0f)), np.nan)
+    hf = np.where(wet_f, 1.0, 0.0)
+    fine = K.RunResult(1.0, np.zeros(1), [ef], [hf], [H0f])
+    H0c = H0f.reshape(n, 4, n, 4).mean(axis=(1, 3))
+    ec = K._block_mean_nan(ef)
+    hc = hf.reshape(n, 4, n, 4).max(axis=(1, 3))
+    run = K.RunResult(1.0, np.zeros(1), [ec], [hc], [H0c])
+    m_run = [K.metrics(fine, run, 0)]
+    model = {"mu": np.zeros(E.NFEAT), "sd": np.ones(E.NFEAT), "W1": np.zeros((E.NFEAT, 64)), "b1": np.zeros(64),
+             "W2": np.zeros((64, 64)), "b2": np.zeros(64), "W3": np.zeros((64, 1)), "b3": np.array([0.05])}
+    mr, t = K.readout_metrics(fine, run, m_run, model)
+    assert len(mr) == 1 and t >= 0
+    assert np.isclose(mr[0]["peak_eta_coarse"], m_run[0]["peak_eta_coarse"] + 0.05)
+    assert mr[0]["peak_eta_sea_coarse"] == m_run[0]["peak_eta_sea_coarse"]
+    for k in K.RUNUP_KEYS:
+        assert mr[0][k] == m_run[0][k]
+    # the highest coarse cell is wetted land
+    zone = H0c < E.NEARSHORE_DEPTH
+    top = np.nanargmax(np.where(zone, ec, np.nan))
+    assert H0c.reshape(-1)[top] < 0

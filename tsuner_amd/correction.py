@@ -460,6 +460,31 @@ def metrics(fine: RunResult, coarse: RunResult, i: int, nearshore_depth: float =
             "runup_coarse_ground": rcg, "runup_rel_err_coarse_ground": rel(rcg, rf)}
 
 
+RUNUP_KEYS = ("runup_fine", "runup_coarse", "runup_rel_err", "runup_coarse_ground", "runup_rel_err_coarse_ground")
+
+
+def readout_metrics(fine: RunResult, run: RunResult, m_run: List[dict], model: dict):
+    """metrics() of every member with the run's eta_max read out by the
+    eta_readout `model` for the eta_max metrics (zone peak, coastal peak,
+    nearshore mean); the runup entries stay those of m_run (the run's own
+    reading).  fine.H0 is the fine scenarios' input bathymetry (copied
+    before the fine run), which a deployment has without running the fine
+    grid.  Returns (per-member metrics, seconds spent in the readout)."""
+    from . import eta_readout
+    import dataclasses
+    t0 = time.perf_counter()
+    ro = dataclasses.replace(run, eta_max=[eta_readout.apply(model, run.eta_max[i], run.hmax[i], run.H0[i],
+                                                             fine.H0[i]) for i in range(len(run.eta_max))])
+    t_ro = time.perf_counter() - t0
+    out = []
+    for i in range(len(run.eta_max)):
+        m = metrics(fine, ro, i)
+        for k in RUNUP_KEYS:
+            m[k] = m_run[i][k]
+        out.append(m)
+    return out, t_ro
+
+
 def save(path: str, net: "nn.Net", scale_out, info: dict):
     info = dict(info, arch=net.arch.as_dict())
     np.savez(path, params=net.params(), scale_out=np.asarray(scale_out, np.float64),
@@ -585,25 +610,10 @@ def evaluate(net, scale_out, seeds: Sequence[int], fine_steps: int = 3000, resid
     out["fallback_policies"] = _fallback_policies(mb, mc, corr, base, fine_per, min_speedup, summary)
 
     def read_out(run, m_run, model):
-        """The run's eta_max read out by `model` for the eta_max metrics
-        (zone peak, coastal peak, nearshore mean); runup stays the run's own
-        reading.  fine.H0 is the fine scenarios' input bathymetry (copied
-        before the fine run), which a deployment has without running the
-        fine grid.  Returns (summary, fallback policies), the readout's own
-        time included in the coarse wall."""
-        from . import eta_readout
+        """readout_metrics of the run, summarised; returns (summary,
+        fallback policies), the readout's own time in the coarse wall."""
         import dataclasses
-        t_ro = time.perf_counter()
-        ro = dataclasses.replace(run, eta_max=[eta_readout.apply(model, run.eta_max[i], run.hmax[i], run.H0[i],
-                                                                 fine.H0[i]) for i in range(n)])
-        t_ro = time.perf_counter() - t_ro
-        mr = []
-        for i in range(n):
-            m = metrics(fine, ro, i)
-            for k in ("runup_fine", "runup_coarse", "runup_rel_err", "runup_coarse_ground",
-                      "runup_rel_err_coarse_ground"):
-                m[k] = m_run[i][k]
-            mr.append(m)
+        mr, t_ro = readout_metrics(fine, run, m_run, model)
         wall_ro = run.wall_s + t_ro
         summ = dict(summary(mr), wall_s=wall_ro, readout_s=t_ro, sims_per_hr=n * 3600 / wall_ro,
                     speedup_vs_fine=fine.wall_s / wall_ro)
