@@ -150,3 +150,39 @@ def test_readout_metrics_move_only_the_front_cells():
     zone = H0c < E.NEARSHORE_DEPTH
     top = np.nanargmax(np.where(zone, ec, np.nan))
     assert H0c.reshape(-1)[top] < 0
+
+
+def test_fallback_policy_calibrates_on_the_first_half():
+    """Fallback thresholds: chosen on the first half of the members as the
+    one with the most fallbacks that keeps >= 8x there, then applied to the
+    second half, where a fallback member costs one fine run and has zero
+    error.  With 20 members, a coarse wall of 1 s for all of them and 10 s
+    of fine run per member, the first half affords one fallback
+    (100 / (0.5 + 10) >= 8, two would give 4.9x)."""
+    n = 20
+    rng = np.random.default_rng(3)
+    resid = rng.permutation(np.arange(n, dtype=np.float64))
+    err = rng.uniform(0.01, 0.2, n)
+    mc = [{"peak_eta_coarse": 1.0 + 0.1 * i, "peak_eta_err": err[i]} for i in range(n)]
+    mb = [{"peak_eta_coarse": 1.0, "peak_eta_err": 2 * err[i]} for i in range(n)]
+    corr = K.RunResult(1.0, np.zeros(n), [], [], [], resid=resid)
+    base = K.RunResult(0.5, np.zeros(n), [], [], [])
+
+    def summary(ms, use_fine=None):
+        e = [0.0 if (use_fine is not None and use_fine[i]) else m["peak_eta_err"] for i, m in enumerate(ms)]
+        return {"peak_eta_err_mean_m": float(np.mean(e))}
+
+    out = K._fallback_policies(mb, mc, corr, base, 10.0, 8.0, summary)
+    pol = out["policies"]["resid_max"]
+    cal, test = resid[:10], resid[10:]
+    tau = np.sort(cal)[-2]                       # exactly one calibration member above it
+    assert pol["threshold"] == tau
+    fb = test > tau
+    assert pol["fallback_frac"] == fb.mean() and pol["scenarios"] == 10
+    np.testing.assert_allclose(pol["peak_eta_err_mean_m"], np.mean(np.where(fb, 0.0, err[10:])))
+    np.testing.assert_allclose(pol["speedup_vs_fine"], 10.0 * 10 / (1.0 * 10 / n + 10.0 * fb.sum()))
+    # an indicator that is NaN never sends a member to the fine solver
+    corr_nan = K.RunResult(1.0, np.zeros(n), [], [], [], resid=np.full(n, np.nan))
+    pol = K._fallback_policies(mb, mc, corr_nan, base, 10.0, 8.0, summary)["policies"]["resid_max"]
+    assert pol["fallback_frac"] == 0.0
+    np.testing.assert_allclose(pol["peak_eta_err_mean_m"], np.mean(err[10:]))
