@@ -143,3 +143,33 @@ def test_silu_saturates_without_warnings():
     with np.errstate(all="raise"):
         y = E._silu(np.array([-1000.0, -50.0, 0.0, 50.0]))
     assert y[0] == 0 and abs(y[1]) < 1e-18 and y[2] == 0 and y[3] == 50.0
+
+
+def test_samples_targets_and_peaks():
+    """samples() on synthetic runs: one row per corrected cell with target
+    fine block mean - coarse eta_max (NaN where the fine run never wetted the
+    block), the member index, and per member the fine run's zone peak."""
+    from tsuner_amd import correction as K
+    fines, coarses = [], []
+    for seed in (7, 8):
+        eta, hmax, H0c, H0f = _case(seed=seed)
+        n = eta.shape[0]
+        rng = np.random.default_rng(seed)
+        ef = np.repeat(np.repeat(np.where(np.isfinite(eta), eta - 0.03, np.nan), 4, 0), 4, 1)
+        ef = ef + rng.normal(0, 0.001, ef.shape)
+        ef[:4, :4] = np.nan                       # one block the fine run never wetted
+        fines.append((ef, H0f))
+        coarses.append((eta, hmax, H0c))
+    fine = K.RunResult(1.0, np.zeros(2), [f[0] for f in fines], [np.zeros_like(f[0]) for f in fines],
+                       [f[1] for f in fines])
+    coarse = K.RunResult(1.0, np.zeros(2), [c[0] for c in coarses], [c[1] for c in coarses],
+                         [c[2] for c in coarses])
+    X, y, g, peak = E.samples(fine, coarse)
+    assert X.shape[1] == E.NFEAT and len(X) == len(y) == len(g) and set(np.unique(g)) <= {0, 1}
+    for k in (0, 1):
+        idx, Xk = E.features(*coarses[k], fines[k][1])
+        np.testing.assert_array_equal(X[g == k], Xk)
+        efb = K._block_mean_nan(fines[k][0])
+        np.testing.assert_array_equal(y[g == k], efb.reshape(-1)[idx] - Xk[:, 0])
+        zone = coarses[k][2] < E.NEARSHORE_DEPTH
+        assert peak[k] == np.nanmax(efb[zone])
