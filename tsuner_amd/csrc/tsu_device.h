@@ -600,7 +600,7 @@ struct EdgeRP {
     Vel<R> u0, u1;
 };
 
-// Hydrostatic depths (VolumeSolver.cu:24-27).
+// Hydrostatic depths (VolumeSolver.cu:37-40).
 template <typename R>
 __device__ __forceinline__ void rp_depths(R hl, R zl, R hr, R zr, EdgeRP<R>& e)
 {
@@ -611,8 +611,9 @@ __device__ __forceinline__ void rp_depths(R hl, R zl, R hr, R zr, EdgeRP<R>& e)
     e.h1 = rmax(hr - zr + zmH, (R)0);
 }
 
-// Velocities and concentrations of both sides.  Returns true (uniform over
-// the wave) when no lane has a velocity or a concentration.
+// Velocities and concentrations of both sides (VolumeSolver.cu:32-35).
+// Returns true (uniform over the wave) when no lane has a velocity or a
+// concentration.
 template <typename R, bool SED>
 __device__ __forceinline__ bool rp_velocities(R hl, R qnl, R qtl, R hcl, R hr, R qnr, R qtr, R hcr, EdgeRP<R>& e)
 {
